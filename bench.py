@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Throughput of the propagation path on one MI355X.
+
+Workload (default): 2048 cosZ in [-1, 1] x 2048 energies log-spaced 0.05-1000 GeV on the
+12-shell jittered PREM-like Earth; one step = the four variants nu/nubar x normal/inverted
+ordering, i.e. four calculateProbabilities of the full grid (16.8 M pairs), inputs already
+resident in HBM, results left in HBM (SoA, FP32 for the mixed kernel).
+
+Prints one JSON line.  value = pairs/s of the whole step sequence; roofline = algorithmic
+flops of the reference algorithm per launch / average duration of the per-pair kernel
+(HIP events on its stream); cpu_baseline = the reference CpuPropagator<double> (or the
+oracle restatement if the reference build is absent) on the same four grids with all
+host threads of the box's share.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_props(grids, precision):
+    from cudaprob3_amd.propagator import HipPropagatorSingle
+    g0 = grids[0]
+    p = HipPropagatorSingle(0, len(g0.cosines), len(g0.energies), precision=precision)
+    p.setEnergyList(g0.energies)
+    p.setCosineList(g0.cosines)
+    p.setDensity(g0.radii, g0.rhos)
+    p.setProductionHeight(g0.h_km)
+    return p
+
+
+def run_variant(p, g, wait=False):
+    p.setMNSMatrix(*g.params[:4])
+    p.setNeutrinoMasses(*g.params[4:])
+    p.calculateProbabilitiesAsync(g.antinu)
+    if wait:
+        p.waitForCompletion()
+
+
+def time_steps(p, grids, steps, warmup, sync):
+    for _ in range(warmup):
+        for g in grids:
+            run_variant(p, g)
+        p.waitForCompletion()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for g in grids:
+            run_variant(p, g)
+        p.waitForCompletion()
+    sync()
+    return time.perf_counter() - t0
+
+
+def kernel_ms(p, grids, reps):
+    """Average GPU duration of the per-pair kernel and of all kernels, per launch."""
+    pairs, total = [], []
+    for _ in range(reps):
+        for g in grids:
+            run_variant(p, g, wait=True)
+            t, k = p.last_compute_times()
+            total.append(t)
+            pairs.append(k)
+    return float(np.mean(pairs)), float(np.mean(total))
+
+
+def accuracy(grids, p_mixed):
+    """max / rms |dP| of the mixed kernel against the FP64 kernel (pinned to the reference
+    within 1e-12 by the tests) over the full grids."""
+    p64 = make_props(grids, "fp64")
+    worst, sq, n = 0.0, 0.0, 0
+    for g in grids:
+        run_variant(p64, g, wait=True)
+        run_variant(p_mixed, g, wait=True)
+        a = p64.getProbabilities().copy()
+        p64.resultsResideOnHost = False
+        b = p_mixed.getProbabilities().astype(np.float64)
+        p_mixed.resultsResideOnHost = False
+        d = np.abs(a - b)
+        worst = max(worst, float(d.max()))
+        sq += float((d * d).sum())
+        n += d.size
+    p64.close()
+    return worst, float(np.sqrt(sq / n))
+
+
+def cpu_baseline(grids, threads):
+    import oracle
+    kind = "reference" if oracle.ref_available() else "port"
+    fn = oracle.ref_propagate if kind == "reference" else oracle.propagate
+    t0 = time.perf_counter()
+    pairs = 0
+    for g in grids:
+        fn(g.antinu, g.cosines, g.energies, g.radii, g.rhos, g.h_km, g.params, threads=threads)
+        pairs += g.n_pairs
+    dt = time.perf_counter() - t0
+    return {"value": pairs / dt, "unit": "pairs/s", "cores": threads, "kind": kind,
+            "sample": f"one full step: {len(grids)} grids of {len(grids[0].cosines)}x{len(grids[0].energies)} "
+                      f"pairs, CpuPropagator<double> algorithm, OpenMP dynamic over cosines, {dt:.1f} s"}
+
+
+def traffic_from_profile(workload, precision):
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{precision}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64", "fp64_plain"])
+    ap.add_argument("--n", type=int, default=2048, help="grid edge (cosines = energies = n)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the other precisions' throughput")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU "
+              f"path (asked for --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from cudaprob3_amd import metrics as M
+    from cudaprob3_amd import workloads as W
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU (cuda:0)", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.set_device(0)
+    sync = torch.cuda.synchronize
+
+    grids = W.jitter12_variants(args.n)
+    workload = f"jitter12_{args.n}"
+    pairs_per_step = sum(g.n_pairs for g in grids)
+    log(f"[bench] workload {workload}: {len(grids)} variants x {grids[0].n_pairs} pairs, precision {args.precision}")
+
+    p = make_props(grids, args.precision)
+    dt = time_steps(p, grids, args.steps, args.warmup, sync)
+    value = args.steps * pairs_per_step / dt
+    log(f"[bench] {args.steps} steps in {dt:.4f} s -> {value:.4e} pairs/s")
+
+    k_ms, total_ms = kernel_ms(p, grids, reps=3)
+    ml = M.maxlayers_of(grids[0].cosines, grids[0].radii, grids[0].rhos)
+    flops = M.grid_flops(ml, len(grids[0].energies))
+    achieved = flops / (k_ms * 1e-3) / 1e12
+    peak = M.PEAK_FP32_VALU_TFLOPS if args.precision == "mixed" else M.PEAK_FP64_VALU_TFLOPS
+    out_bytes = (4 if args.precision == "mixed" else 8) * 9 * grids[0].n_pairs
+    log(f"[bench] per-pair kernel {k_ms:.4f} ms, all kernels {total_ms:.4f} ms per launch; "
+        f"{flops / 1e9:.2f} GFLOP algorithmic -> {achieved:.1f} TFLOP/s; output {out_bytes / (k_ms * 1e-3) / 1e9:.1f} GB/s")
+
+    result = {
+        "metric": "pairs/s: exact mixed-FP32 vs surrogate at max|dP|<=1e-3 vs FP64, 2048^2 grid",
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 eigenvalues and phase reduction)" if args.precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{workload}: {args.n} cosZ in [-1,1] x {args.n} E log 0.05-1000 GeV, "
+                               "12 jittered PREM shells, h=22 km, nu/nubar x NO/IO (4 grids per step)",
+                   "precision": args.precision, "pairs_per_step": pairs_per_step},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak,
+                     "traffic": traffic_from_profile(workload, args.precision),
+                     "kernel": "pairs_kernel", "kernel_ms": k_ms, "all_kernels_ms": total_ms,
+                     "algorithmic_flops_per_launch": flops,
+                     "output_GBps": out_bytes / (k_ms * 1e-3) / 1e9},
+    }
+
+    if not args.no_accuracy and args.precision == "mixed":
+        worst, rms = accuracy(grids, p)
+        result["accuracy_vs_fp64"] = {"max_abs_dP": worst, "rms_dP": rms}
+        log(f"[bench] mixed vs fp64: max|dP| {worst:.3e} rms {rms:.3e}")
+
+    if not args.no_extra:
+        extra = {}
+        for prec in ("fp64", "fp64_plain"):
+            if prec == args.precision:
+                continue
+            q = make_props(grids, prec)
+            steps = 2
+            dtq = time_steps(q, grids, steps, 1, sync)
+            extra[prec] = steps * pairs_per_step / dtq
+            log(f"[bench] {prec}: {extra[prec]:.4e} pairs/s")
+            q.close()
+        result["other_precisions_pairs_per_s"] = extra
+
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"[bench] CPU baseline with {threads} threads ...")
+        result["cpu_baseline"] = cpu_baseline(grids, threads)
+        log(f"[bench] CPU baseline {result['cpu_baseline']['value']:.4e} pairs/s")
+    p.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

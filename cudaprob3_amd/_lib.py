@@ -1,0 +1,87 @@
+"""ctypes binding of libcp3hip.so (C ABI declared in include/cp3hip.h).
+
+The library is built in-tree by ``__graft_entry__.build()``.  Loading fails loudly if it
+is missing: there is no CPU fallback for the propagation path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcp3hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "cp3hip.h")
+
+CP3H_OK = 0
+CP3H_FP64_PLAIN = 0
+CP3H_FP64 = 1
+CP3H_MIXED = 2
+PRECISIONS = {"fp64_plain": CP3H_FP64_PLAIN, "fp64": CP3H_FP64, "mixed": CP3H_MIXED}
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_float_p = ctypes.POINTER(ctypes.c_float)
+_handle = ctypes.c_void_p
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "cp3h_device_count": (ctypes.c_int, [_c_int_p]),
+    "cp3h_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(_handle)]),
+    "cp3h_destroy": (ctypes.c_int, [_handle]),
+    "cp3h_set_energies_f64": (ctypes.c_int, [_handle, _c_double_p, ctypes.c_int]),
+    "cp3h_set_energies_f32": (ctypes.c_int, [_handle, _c_float_p, ctypes.c_int]),
+    "cp3h_set_cosines_f64": (ctypes.c_int, [_handle, _c_double_p, _c_int_p, ctypes.c_int]),
+    "cp3h_set_cosines_f32": (ctypes.c_int, [_handle, _c_float_p, _c_int_p, ctypes.c_int]),
+    "cp3h_set_maxlayers": (ctypes.c_int, [_handle, _c_int_p, ctypes.c_int]),
+    "cp3h_set_density_f64": (ctypes.c_int, [_handle, _c_double_p, _c_double_p, ctypes.c_int]),
+    "cp3h_set_density_f32": (ctypes.c_int, [_handle, _c_float_p, _c_float_p, ctypes.c_int]),
+    "cp3h_set_params_f64": (ctypes.c_int, [_handle, _c_double_p, _c_double_p]),
+    "cp3h_set_params_f32": (ctypes.c_int, [_handle, _c_float_p, _c_float_p]),
+    "cp3h_compute_async": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double]),
+    "cp3h_wait": (ctypes.c_int, [_handle]),
+    "cp3h_compute": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double]),
+    "cp3h_copy_results_f64": (ctypes.c_int, [_handle, _c_double_p, ctypes.c_size_t]),
+    "cp3h_copy_results_f32": (ctypes.c_int, [_handle, _c_float_p, ctypes.c_size_t]),
+    "cp3h_device_results": (ctypes.c_int, [_handle, ctypes.POINTER(ctypes.c_void_p), _c_int_p]),
+    "cp3h_stream": (ctypes.c_int, [_handle, ctypes.POINTER(ctypes.c_void_p)]),
+    "cp3h_last_compute_ms": (ctypes.c_int, [_handle, _c_float_p, _c_float_p]),
+    "cp3h_last_error": (ctypes.c_char_p, []),
+}
+
+_LIB: ctypes.CDLL | None = None
+
+
+class Cp3hError(RuntimeError):
+    """A libcp3hip.so call returned a nonzero status."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+
+
+def load() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() first "
+                               "(the propagation path has no CPU fallback)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def check(status: int) -> None:
+    if status != CP3H_OK:
+        msg = load().cp3h_last_error().decode()
+        raise Cp3hError(status, msg)
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    check(load().cp3h_device_count(ctypes.byref(n)))
+    return n.value

@@ -1,0 +1,77 @@
+// Shared definitions of the MI355X propagation library (device and host).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace cp3 {
+
+// Physical constants of the reference (constants.hpp:28-40, physics.hpp:442).
+constexpr double kGF = 1.52588e-4;          // 2*sqrt(2)*G_F*N_A scale, eV^2 per (GeV g/cm^3)
+constexpr double kKm2Cm = 1.0e5;
+constexpr double kREarth = 6371.0;          // km
+constexpr double kREarthCm = kREarth * kKm2Cm;
+constexpr double kYe = 0.5;                 // electrons per nucleon
+constexpr double kLoE = 2.534;              // (1/2)(1/(hbar c)) in GeV/(eV^2 km)
+constexpr double kPi = 3.14159265358979323846;  // M_PI
+
+// Everything a launch needs about the oscillation hypothesis, passed by value as a
+// kernel argument (no process-global device symbols, so concurrent propagators with
+// different parameters never race).  U is row-major, (re, im) interleaved.
+struct OscParams {
+    double U[18];      // PMNS matrix U(i,j) = (U[2(3i+j)], U[2(3i+j)+1])
+    double DM[9];      // DM(i,j) = m_i^2 - m_j^2 (with the reference's degeneracy nudge)
+    int order[3];      // matter root k that continues vacuum mass i
+    int antinu;        // 1: matter potential sign flipped (the only nu/nubar difference)
+};
+
+constexpr int kWave = 64;            // CDNA wavefront
+constexpr int kBlock = 256;          // 4 wavefronts per workgroup
+constexpr int kXcds = 8;             // MI355X accelerator complex dies
+
+// Workgroup id -> logical id so that consecutive logical ids run on the same XCD
+// (hardware hands workgroups to XCDs round robin).  grid must be a multiple of kXcds.
+__device__ inline unsigned xcd_logical_block(unsigned b, unsigned nblocks) {
+    const unsigned per = nblocks / kXcds;
+    return (b % kXcds) * per + (b / kXcds);
+}
+
+inline unsigned round_up(unsigned x, unsigned m) { return (x + m - 1) / m * m; }
+
+// ---- launchers (defined in the .hip translation units) --------------------------------
+
+struct GridDesc {
+    const double* cosines;     // [n_c]
+    const double* energies;    // [n_e]
+    const int* maxlayers;      // [n_c], each in [0, rows]
+    const double* radii;       // [rows] descending, radii[0] = outer radius
+    const double* rhos;        // [rows] density of the shell whose outer radius is radii[i]
+    int n_c, n_e, rows;
+    double h_cm;               // production height in cm
+};
+
+// Plain FP64 kernel: the reference's per-pair Barger step in double.  out: SoA [9][n_c][n_e].
+hipError_t launch_plain_f64(const GridDesc& g, const OscParams& p, double* out, hipStream_t s);
+
+// Layer slots: layer i of a path (0 = atmosphere, i >= 1 = the i-th shell from the surface)
+// uses density slot i; maxlayers <= rows (== rows only for a model without an r = 0 row),
+// so rows + 1 slots cover every path.
+
+// Per-cosine layer lengths in km: geom[ic*(rows+1) + i], i = 0..maxlayers[ic].
+hipError_t launch_geometry(const GridDesc& g, double* geom, hipStream_t s);
+
+// Per-(shell, energy) spectrum: flavour-basis projectors and phase coefficients.
+// Layout: Q[(s*27 + f)*ne_pad + e], R[(s*3 + k)*ne_pad + e], s = 0..rows (s = 0 is vacuum,
+// s >= 1 has the density rhos[s-1]).
+// Q is float or double; R holds -2.534*lambda_k (double table) or -2.534*lambda_k/(2 pi E)
+// (float table: phase in revolutions).
+hipError_t launch_spectrum_f32(const GridDesc& g, const OscParams& p, int ne_pad, float* Q, double* R, hipStream_t s);
+hipError_t launch_spectrum_f64(const GridDesc& g, const OscParams& p, int ne_pad, double* Q, double* R, hipStream_t s);
+
+// Table-driven per-pair kernels.
+hipError_t launch_pairs_mixed(const GridDesc& g, int ne_pad, const double* geom, const float* Q,
+                              const double* R, float* out, hipStream_t s);
+hipError_t launch_pairs_f64(const GridDesc& g, int ne_pad, const double* geom, const double* Q,
+                            const double* R, double* out, hipStream_t s);
+
+}  // namespace cp3

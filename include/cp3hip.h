@@ -1,0 +1,117 @@
+/* C ABI of libcp3hip.so -- the MI355X (gfx950) propagation library.
+ *
+ * It replaces the device side of the reference's single-GPU propagator
+ * (cudapropagator.cuh:39-251, class CudaPropagatorSingle) and the CUDA kernel it
+ * launches (physics.hpp:583-742).  Host-side set-up -- the PMNS matrix, the mass
+ * differences, the density flip, coslimit and maxlayers -- stays with the caller's
+ * Propagator<FLOAT_T> (include/cudaprob3/propagator.hpp, following propagator.hpp:117-307),
+ * which hands the results across this boundary as plain arrays.
+ *
+ * Conventions: every function returns CP3H_OK (0) or a nonzero status; the message of
+ * the last failure on the calling thread is cp3h_last_error().  Arrays are host
+ * pointers unless stated otherwise.  A handle is not thread-safe; distinct handles are
+ * independent (all parameters travel per launch, no process-global device state).
+ * Results are laid out SoA: result[t * n_cosines * n_energies + ic * n_energies + ie],
+ * t = in*3 + out (types.hpp:23-27), as the reference GPU path writes them
+ * (physics.hpp:689-691).
+ */
+#ifndef CP3HIP_H
+#define CP3HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cp3h_propagator cp3h_propagator;
+
+enum cp3h_status {
+    CP3H_OK = 0,
+    CP3H_ERR_ARGUMENT = 1, /* bad size, index, pointer or enum */
+    CP3H_ERR_STATE = 2,    /* call order violated (e.g. compute before inputs are set) */
+    CP3H_ERR_DEVICE = 3,   /* no GPU / bad device id (cudapropagator.cuh:56, 64) */
+    CP3H_ERR_HIP = 4       /* a HIP runtime call failed; message holds hipGetErrorString */
+};
+
+/* Arithmetic of the per-pair kernels. */
+enum cp3h_precision {
+    /* FP64 throughout, the reference's per-pair Barger step with its rounding order
+       (results within 1e-12 of the reference CpuPropagator<double>). */
+    CP3H_FP64_PLAIN = 0,
+    /* FP64, eigen-decomposition hoisted to one per (shell, energy) (spectrum table). */
+    CP3H_FP64 = 1,
+    /* FP32 bulk arithmetic, FP64 eigenvalues and FP64 phase reduction; float results. */
+    CP3H_MIXED = 2
+};
+
+/* Number of visible GPUs.  cudapropagator.cuh:52-54. */
+int cp3h_device_count(int* count);
+
+/* cudapropagator.cuh:50-80 (CudaPropagatorSingle(int id, int n_cosines, int n_energies)).
+   Allocates the device lists and the result buffer on `device`, creates the stream. */
+int cp3h_create(int device, int n_cosines, int n_energies, int precision, cp3h_propagator** out);
+
+/* cudapropagator.cuh:91-94. Frees everything; NULL is accepted. */
+int cp3h_destroy(cp3h_propagator* h);
+
+/* cudapropagator.cuh:148-153 (setEnergyList).  n must equal n_energies; GeV. */
+int cp3h_set_energies_f64(cp3h_propagator* h, const double* energies, int n);
+int cp3h_set_energies_f32(cp3h_propagator* h, const float* energies, int n);
+
+/* cudapropagator.cuh:155-159 and 184-188 (setCosineList + setMaxlayers).  maxlayers[i] is
+   the number of shells crossed at cosines[i] (propagator.hpp:301-307). */
+int cp3h_set_cosines_f64(cp3h_propagator* h, const double* cosines, const int* maxlayers, int n);
+int cp3h_set_cosines_f32(cp3h_propagator* h, const float* cosines, const int* maxlayers, int n);
+
+/* Update maxlayers only (after a density change re-derives them, propagator.hpp:160). */
+int cp3h_set_maxlayers(cp3h_propagator* h, const int* maxlayers, int n);
+
+/* cudapropagator.cuh:132-146 (setDensity): radii in km in DESCENDING order (radii[0] =
+   Earth radius, last = 0) and the density (g/cm^3) of the shell whose outer radius is
+   radii[i], as the base class leaves them after its flip (propagator.hpp:141-147). */
+int cp3h_set_density_f64(cp3h_propagator* h, const double* radii_desc, const double* rhos_desc, int n_rows);
+int cp3h_set_density_f32(cp3h_propagator* h, const float* radii_desc, const float* rhos_desc, int n_rows);
+
+/* physics.hpp:130-194 (setMixMatrix + setMassDifferences): U row-major with (re, im)
+   interleaved (18 values), DM row-major 3x3.  The vacuum root order of
+   physics.hpp:198-253 is derived here on the host in the same precision. */
+int cp3h_set_params_f64(cp3h_propagator* h, const double* U18, const double* DM9);
+int cp3h_set_params_f32(cp3h_propagator* h, const float* U18, const float* DM9);
+
+/* cudapropagator.cuh:191-220 (calculateProbabilitiesAsync): enqueue the propagation on
+   the handle's stream and return.  antineutrino: 0 or 1 (types.hpp:29).
+   production_height_cm: the base class's ProductionHeightinCentimeter. */
+int cp3h_compute_async(cp3h_propagator* h, int antineutrino, double production_height_cm);
+
+/* cudapropagator.cuh:223-226 (waitForCompletion). */
+int cp3h_wait(cp3h_propagator* h);
+
+/* cudapropagator.cuh:162-165 (calculateProbabilities) = compute_async + wait. */
+int cp3h_compute(cp3h_propagator* h, int antineutrino, double production_height_cm);
+
+/* cudapropagator.cuh:229-235 (getResultFromDevice): copy all 9*n_c*n_e results (SoA)
+   to host memory, converting to the requested type. */
+int cp3h_copy_results_f64(cp3h_propagator* h, double* dst, size_t count);
+int cp3h_copy_results_f32(cp3h_propagator* h, float* dst, size_t count);
+
+/* Device-resident results for downstream kernels: device pointer, element size in bytes
+   (8 for the FP64 modes, 4 for CP3H_MIXED).  Valid until the handle is destroyed. */
+int cp3h_device_results(cp3h_propagator* h, const void** ptr, int* elem_bytes);
+
+/* The handle's hipStream_t (as void*), for callers that order their own work after it. */
+int cp3h_stream(cp3h_propagator* h, void** stream);
+
+/* GPU time of the last compute, from HIP events on the handle's stream (waits for it):
+   total_ms covers all of its kernels, pairs_ms only the per-pair propagation kernel
+   (for CP3H_FP64_PLAIN the two are the same kernel).  Either pointer may be NULL. */
+int cp3h_last_compute_ms(cp3h_propagator* h, float* total_ms, float* pairs_ms);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* cp3h_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CP3HIP_H */

@@ -1,0 +1,151 @@
+"""Parity of the MI355X kernels (through libcp3hip.so) with the oracle and the reference's
+golden vectors.
+
+Tolerances (max |dP| over all nine channels):
+  * fp64_plain, fp64 : 1e-12 against the reference / oracle FP64 CPU path;
+  * mixed            : MIXED_TOL against the same FP64 results (FP32 bulk arithmetic,
+                       FP64 eigenvalues and phase reduction).
+"""
+import numpy as np
+import pytest
+
+from conftest import golden_names, load_golden
+from cudaprob3_amd import workloads as W
+from cudaprob3_amd.propagator import HipPropagatorSingle, Neutrino, Antineutrino
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+MIXED_TOL = 1e-5
+TOL = {"fp64_plain": FP64_TOL, "fp64": FP64_TOL, "mixed": MIXED_TOL}
+
+
+def run_gpu(precision, cosines, energies, radii, rhos, params, h_km, antinu, device=0):
+    dtype = np.float64
+    with HipPropagatorSingle(device, len(cosines), len(energies), dtype=dtype, precision=precision) as p:
+        p.setEnergyList(energies)
+        p.setCosineList(cosines)
+        p.setMNSMatrix(*params[:4])
+        p.setNeutrinoMasses(*params[4:])
+        p.setDensity(radii, rhos)
+        p.setProductionHeight(h_km)
+        p.calculateProbabilities(Antineutrino if antinu else Neutrino)
+        return np.ascontiguousarray(p.getProbabilities().transpose(1, 2, 0))   # -> [ic, ie, t]
+
+
+@pytest.mark.parametrize("precision", ["fp64_plain", "fp64", "mixed"])
+@pytest.mark.parametrize("name", golden_names())
+def test_golden(precision, name):
+    g = load_golden(name)
+    print(f"[gpu] golden {name} {precision}", flush=True)
+    P = run_gpu(precision, g["cosines"], g["energies"], g["radii"], g["rhos"], g["params"],
+                float(g["h_km"]), int(g["antinu"]))
+    err = np.abs(P - g["P"]).max()
+    assert err <= TOL[precision], f"{name} {precision}: max|dP| = {err:.3e}"
+
+
+@pytest.mark.parametrize("antinu", [0, 1])
+@pytest.mark.parametrize("precision", ["fp64_plain", "fp64", "mixed"])
+def test_prem4_256_full_grid(oracle_mod, precision, antinu):
+    """256 cosZ x 256 energies, PREM 4 layers: every pair against the oracle."""
+    g = W.prem4_256()
+    print(f"[gpu] prem4_256 {precision} antinu={antinu}", flush=True)
+    ref = oracle_mod.propagate(antinu, g.cosines, g.energies, g.radii, g.rhos, g.h_km, g.params)
+    P = run_gpu(precision, g.cosines, g.energies, g.radii, g.rhos, g.params, g.h_km, antinu)
+    err = np.abs(P - ref)
+    print(f"[gpu] prem4_256 {precision} antinu={antinu}: max {err.max():.3e} rms {np.sqrt((err**2).mean()):.3e}")
+    assert err.max() <= TOL[precision]
+
+
+@pytest.mark.parametrize("n_c,n_e", [(1, 1), (3, 37), (5, 64), (7, 65), (2, 130), (0, 5), (4, 0)])
+@pytest.mark.parametrize("precision", ["fp64_plain", "fp64", "mixed"])
+def test_ragged_and_empty_grids(oracle_mod, precision, n_c, n_e):
+    cos = np.linspace(-1, 1, n_c) if n_c > 1 else np.full(n_c, -0.7)
+    E = np.geomspace(0.2, 40, n_e) if n_e > 1 else np.full(n_e, 2.0)
+    print(f"[gpu] ragged {n_c}x{n_e} {precision}", flush=True)
+    P = run_gpu(precision, cos, E, W.PREM12[:, 0], W.PREM12[:, 1], W.NUFIT, 22.0, 0)
+    assert P.shape == (n_c, n_e, 9)
+    if n_c and n_e:
+        ref = oracle_mod.propagate(0, cos, E, W.PREM12[:, 0], W.PREM12[:, 1], 22.0, W.NUFIT)
+        assert np.abs(P - ref).max() <= TOL[precision]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_jitter12_2048_full_size(oracle_mod, precision):
+    """The 2048 x 2048 jittered 12-shell workload, all four nu/nubar x NO/IO variants:
+    strided rows against the oracle, unitarity over the full grid."""
+    for g in W.jitter12_variants(2048):
+        print(f"[gpu] jitter12_2048 {precision} antinu={g.antinu} dm32={g.params[5]}", flush=True)
+        P = run_gpu(precision, g.cosines, g.energies, g.radii, g.rhos, g.params, g.h_km, g.antinu)
+        rows = np.arange(0, 2048, 37)
+        ref = oracle_mod.propagate(g.antinu, g.cosines[rows], g.energies, g.radii, g.rhos, g.h_km, g.params)
+        err = np.abs(P[rows] - ref)
+        print(f"[gpu]   max {err.max():.3e} rms {np.sqrt((err**2).mean()):.3e}")
+        assert err.max() <= TOL[precision]
+        M = P.reshape(2048, 2048, 3, 3)
+        uni = max(np.abs(M.sum(axis=3) - 1).max(), np.abs(M.sum(axis=2) - 1).max())
+        assert uni <= 10 * TOL[precision]
+
+
+def test_api_errors_and_state():
+    with pytest.raises(RuntimeError, match="not available"):
+        HipPropagatorSingle(999, 4, 4)
+    p = HipPropagatorSingle(4, 4)
+    p.setEnergyList(np.geomspace(1, 10, 4))
+    p.setCosineList(np.linspace(-1, 0, 4))
+    p.setMNSMatrix(*W.NUFIT[:4])
+    p.setNeutrinoMasses(*W.NUFIT[4:])
+    with pytest.raises(RuntimeError, match="production height was not set"):
+        p.calculateProbabilities(Neutrino)
+    p.setProductionHeight(22.0)
+    with pytest.raises(RuntimeError, match="density was not set"):
+        p.calculateProbabilities(Neutrino)
+    p.setDensity(W.PREM4[:, 0], W.PREM4[:, 1])
+    p.calculateProbabilities(Neutrino)
+    with pytest.raises(RuntimeError, match="Invalid indices"):
+        p.getProbability(4, 0, 0)
+    v = p.getProbability(3, 3, 8)
+    assert 0.0 <= v <= 1.0
+    assert p.last_compute_ms() > 0.0
+    p.close()
+    with pytest.raises(RuntimeError, match="moved from"):
+        p.calculateProbabilities(Neutrino)
+
+
+def test_independent_handles_do_not_share_parameters(oracle_mod):
+    """Two propagators with different hypotheses, launched back to back without waiting,
+    keep their own parameters (the reference's process-global constants would race)."""
+    g = W.prem4_256()
+    cos, E = g.cosines[::4], g.energies[::4]
+    hyp = [W.NUFIT, (0.6, 0.14, 0.9, 0.3, 7.0e-5, -2.4e-3)]
+    props = []
+    for h in hyp:
+        p = HipPropagatorSingle(len(cos), len(E), precision="fp64")
+        p.setEnergyList(E); p.setCosineList(cos); p.setMNSMatrix(*h[:4]); p.setNeutrinoMasses(*h[4:])
+        p.setDensity(g.radii, g.rhos); p.setProductionHeight(22.0)
+        props.append(p)
+    for p in props:
+        p.calculateProbabilitiesAsync(Neutrino)
+    for p, h in zip(props, hyp):
+        p.waitForCompletion()
+        ref = oracle_mod.propagate(0, cos, E, g.radii, g.rhos, 22.0, h)
+        assert np.abs(p.getProbabilities().transpose(1, 2, 0) - ref).max() <= FP64_TOL
+        p.close()
+
+
+def test_float32_api_runs_mixed(oracle_mod):
+    g = W.prem4_256()
+    cos, E = g.cosines[::8].astype(np.float32), g.energies[::8].astype(np.float32)
+    with HipPropagatorSingle(0, len(cos), len(E), dtype=np.float32) as p:
+        assert p.precision == "mixed"
+        p.setEnergyList(E); p.setCosineList(cos)
+        p.setMNSMatrix(*W.NUFIT[:4]); p.setNeutrinoMasses(*W.NUFIT[4:])
+        p.setDensity(g.radii, g.rhos); p.setProductionHeight(22.0)
+        p.calculateProbabilities(Neutrino)
+        P = p.getProbabilities().transpose(1, 2, 0)
+        assert P.dtype == np.float32
+        # the FP64 oracle on the same float32 inputs, with float32-rounded parameters
+        params = [float(np.float32(x)) for x in W.NUFIT]
+        ref = oracle_mod.propagate(0, cos.astype(np.float64), E.astype(np.float64), g.radii.astype(np.float32),
+                                   g.rhos.astype(np.float32), 22.0, params)
+        assert np.abs(P - ref).max() <= 1e-4

@@ -1,0 +1,191 @@
+"""Pins of the CPU oracle (oracle/cp3_oracle.cpp), on the CPU.
+
+* bit-exact against the golden fixtures written by the compiled reference;
+* bit-exact against the compiled reference itself on seeded random cases (when built);
+* against closed forms: the vacuum 3-flavour formula, the layer matrix
+  U exp(-i 2.534 (L/E) H) U^dagger (scipy expm), the eigenvalues of H (numpy eigvalsh),
+  unitarity, the chord sum rule and brute-force shell counting;
+* planted mistakes (textbook antineutrino conjugation, a transposed output index) fail.
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from conftest import golden_names, load_golden
+from cudaprob3_amd import workloads as W
+
+KGF = 1.52588e-4
+LOE = 2.534
+R = 6371.0
+
+
+def pdg_U(t12, t13, t23, d):
+    s12, s13, s23, c12, c13, c23 = np.sin(t12), np.sin(t13), np.sin(t23), np.cos(t12), np.cos(t13), np.cos(t23)
+    e = np.exp(-1j * d)
+    return np.array([
+        [c12 * c13, s12 * c13, s13 * e],
+        [-s12 * c23 - c12 * s23 * s13 * np.conj(e), c12 * c23 - s12 * s23 * s13 * np.conj(e), s23 * c13],
+        [s12 * s23 - c12 * c23 * s13 * np.conj(e), -c12 * s23 - s12 * c23 * s13 * np.conj(e), c23 * c13]])
+
+
+def mass_basis_H(U, dm21, dm32, E, rho_eff, antinu):
+    """2EH in the mass basis (eV^2): diag(0, dm21, dm31) + a w w^dagger, w = conj(U_e.),
+    a = +/- 2 sqrt2 GF Ne E (sign flip only for antineutrinos; U is not conjugated)."""
+    a = (-1.0 if antinu else 1.0) * KGF * E * rho_eff
+    w = np.conj(U[0, :])
+    return np.diag([0.0, dm21, dm21 + dm32]).astype(complex) + a * np.outer(w, np.conj(w))
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_oracle_bit_exact_vs_golden(oracle_mod, name):
+    g = load_golden(name)
+    P = oracle_mod.propagate(int(g["antinu"]), g["cosines"], g["energies"], g["radii"], g["rhos"],
+                             float(g["h_km"]), g["params"], threads=2)
+    np.testing.assert_array_equal(P, g["P"])
+
+
+def test_oracle_bit_exact_vs_reference_build(oracle_mod):
+    if not oracle_mod.ref_available():
+        pytest.skip("oracle/_ref/libcp3ref.so not built (needs the reference tree)")
+    rng = np.random.Generator(np.random.PCG64(11))
+    for case in range(6):
+        p = W.random_params(rng)
+        radii, rhos = W.random_earth(rng, int(rng.integers(2, 30)))
+        cos = np.sort(rng.uniform(-1, 1, 24))
+        E = np.sort(rng.uniform(0.05, 200, 20))
+        antinu = case % 2
+        a = oracle_mod.propagate(antinu, cos, E, radii, rhos, 22.0, p, threads=2)
+        b = oracle_mod.ref_propagate(antinu, cos, E, radii, rhos, 22.0, p, threads=2)
+        np.testing.assert_array_equal(a, b)
+
+
+def test_host_setup_matches_reference(oracle_mod):
+    """U, DM, maxlayers of the oracle equal the reference's host state bit for bit."""
+    if not oracle_mod.ref_available():
+        pytest.skip("reference build missing")
+    import ctypes
+    lib = oracle_mod.ref_lib()
+    cos = np.linspace(-1, 1, 41)
+    radii, rhos = W.PREM12[:, 0].copy(), W.PREM12[:, 1].copy()
+    U, D = np.empty(18), np.empty(9)
+    ml = np.empty(41, np.int32)
+    cl, rd, rh = np.empty(13), np.empty(13), np.empty(13)
+    hcm = ctypes.c_double()
+    dp = ctypes.POINTER(ctypes.c_double)
+    st = lib.ref_setup(cos.ctypes.data_as(dp), 41, radii.ctypes.data_as(dp), rhos.ctypes.data_as(dp), 13,
+                       ctypes.c_double(22.0), *[ctypes.c_double(x) for x in W.NUFIT],
+                       U.ctypes.data_as(dp), D.ctypes.data_as(dp), ml.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                       cl.ctypes.data_as(dp), rd.ctypes.data_as(dp), rh.ctypes.data_as(dp), ctypes.byref(hcm))
+    assert st == 0
+    np.testing.assert_array_equal(oracle_mod.mns_matrix(*W.NUFIT[:4]), U)
+    np.testing.assert_array_equal(oracle_mod.mass_matrix(*W.NUFIT[4:]), D)
+    rd2, rh2, cl2 = oracle_mod.set_density(radii, rhos)
+    np.testing.assert_array_equal(rd2, rd)
+    np.testing.assert_array_equal(rh2, rh)
+    np.testing.assert_array_equal(cl2, cl)
+    np.testing.assert_array_equal(oracle_mod.maxlayers(cos, cl2), ml)
+
+
+def test_vacuum_closed_form(oracle_mod):
+    """cos >= 0: one vacuum layer of length PathLength; P = |sum_k U_bk U*_ak e^{-i 2.534 m_k L/E}|^2."""
+    for params in (W.NUFIT, (0.59, 0.15, 0.80, 1.1, 7.5e-5, -2.45e-3)):
+        U = pdg_U(*params[:4])
+        m = np.array([0.0, params[4], params[4] + params[5]])
+        cos = np.array([0.0, 0.2, 0.7, 1.0])
+        E = np.geomspace(0.1, 50, 9)
+        h_cm = 22.0 * 1e5
+        Rcm = R * 1e5
+        for antinu in (0, 1):
+            P = oracle_mod.propagate(antinu, cos, E, W.PREM4[:, 0], W.PREM4[:, 1], 22.0, params, threads=1)
+            for ic, c in enumerate(cos):
+                Lkm = (np.sqrt((Rcm + h_cm) ** 2 - Rcm ** 2 * (1 - c * c)) - Rcm * c) / 1e5
+                for ie, e in enumerate(E):
+                    A = U @ np.diag(np.exp(-1j * LOE * m * Lkm / e)) @ U.conj().T
+                    expect = (np.abs(A) ** 2).T.reshape(9)     # t = in*3+out -> |A[out][in]|^2
+                    np.testing.assert_allclose(P[ic, ie], expect, atol=2e-13)
+
+
+@pytest.mark.parametrize("antinu", [0, 1])
+@pytest.mark.parametrize("params", [W.NUFIT, (0.57, 0.16, 0.75, 5.0, 7.0e-5, -2.6e-3)])
+def test_layer_matrix_is_exponential(oracle_mod, antinu, params):
+    """A_layer = U expm(-i 2.534 (L/E) H) U^dagger for a constant-density layer."""
+    U18 = oracle_mod.mns_matrix(*params[:4])
+    DM9 = oracle_mod.mass_matrix(*params[4:])
+    U = (U18[0::2] + 1j * U18[1::2]).reshape(3, 3)
+    np.testing.assert_allclose(U, pdg_U(*params[:4]), atol=1e-15)
+    for E, rho, L in [(0.3, 1.65, 900.0), (2.0, 6.5, 3000.0), (8.0, 5.65, 2400.0), (30.0, 2.5, 5000.0)]:
+        A = oracle_mod.transition_matrix(U18, DM9, antinu, E, rho, L)
+        H = mass_basis_H(U, params[4], params[5], E, rho, antinu)
+        expect = U @ scipy.linalg.expm(-1j * LOE * (L / E) * H) @ U.conj().T
+        np.testing.assert_allclose(A, expect, atol=1e-11)
+        np.testing.assert_allclose(A @ A.conj().T, np.eye(3), atol=1e-12)
+
+
+def test_matter_eigenvalues(oracle_mod):
+    U18 = oracle_mod.mns_matrix(*W.NUFIT[:4])
+    DM9 = oracle_mod.mass_matrix(*W.NUFIT[4:])
+    U = pdg_U(*W.NUFIT[:4])
+    for antinu in (0, 1):
+        for E in (0.1, 1.0, 6.0, 40.0, 500.0):
+            for rho in (0.0, 1.65, 6.5):
+                mm, mv = oracle_mod.matter_masses(U18, DM9, E, rho, antinu)
+                lam = np.sort(mv[:, 0])                # mv[k][0] = m_k - DM(0,0) = m_k
+                H = mass_basis_H(U, W.NUFIT[4], W.NUFIT[5], E, rho, antinu)
+                ev = np.linalg.eigvalsh(H)
+                scale = max(1.0, np.abs(ev).max() / 2.5e-3)
+                np.testing.assert_allclose(lam, ev, atol=2e-16 * 1e3 * scale, rtol=1e-11)
+                if rho == 0.0:   # vacuum: the root order reproduces DM(k,0)
+                    np.testing.assert_allclose(mv[:, 0], [0.0, W.NUFIT[4], W.NUFIT[4] + W.NUFIT[5]], atol=1e-17)
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_unitarity_of_golden(name):
+    P = load_golden(name)["P"].reshape(-1, 3, 3)
+    np.testing.assert_allclose(P.sum(axis=2), 1.0, atol=1e-12)
+    np.testing.assert_allclose(P.sum(axis=1), 1.0, atol=1e-12)
+
+
+def test_chord_sum_rule(oracle_mod):
+    rd, rh, cl = oracle_mod.set_density(W.PREM12[:, 0], W.PREM12[:, 1])
+    h_cm = 22.0e5
+    for c in np.linspace(-1, -0.01, 37):
+        L = int(oracle_mod.maxlayers([c], cl)[0])
+        d = oracle_mod.layer_distances(rd, L, c, h_cm)
+        Rcm = R * 1e5
+        path = np.sqrt((Rcm + h_cm) ** 2 - Rcm ** 2 * (1 - c * c)) - Rcm * c
+        total = d[0] + 2 * d[1:L].sum() + d[L]
+        assert abs(total - path) <= 1e-7 * path
+        assert np.all(d[1:] >= 0)
+
+
+def test_maxlayers_brute_force(oracle_mod):
+    rng = np.random.Generator(np.random.PCG64(5))
+    for _ in range(20):
+        radii, rhos = W.random_earth(rng, int(rng.integers(1, 40)))
+        rd, _, cl = oracle_mod.set_density(radii, rhos)
+        cos = rng.uniform(-1, 1, 200)
+        got = oracle_mod.maxlayers(cos, cl)
+        b = R * np.sqrt(1 - cos ** 2)
+        brute = np.where(cos < 0, 1 + (rd[None, 1:] > b[:, None]).sum(axis=1), 0)
+        np.testing.assert_array_equal(got, brute)
+    assert oracle_mod.maxlayers([-1.0], cl)[0] == len(rd) - 1      # model with an r = 0 row
+
+
+def test_planted_mistakes_are_caught(oracle_mod):
+    """The closed form distinguishes the reference's antineutrino convention (matter sign
+    flip, U not conjugated) and the output index t = in*3+out from plausible mistakes."""
+    params = (0.58, 0.15, 0.72, 1.3, 7.4e-5, 2.5e-3)
+    U18 = oracle_mod.mns_matrix(*params[:4])
+    DM9 = oracle_mod.mass_matrix(*params[4:])
+    U = pdg_U(*params[:4])
+    E, rho, L = 3.0, 5.0, 4000.0
+    A = oracle_mod.transition_matrix(U18, DM9, 1, E, rho, L)
+    Hright = mass_basis_H(U, params[4], params[5], E, rho, 1)
+    Hconj = mass_basis_H(np.conj(U), params[4], params[5], E, rho, 1)
+    right = U @ scipy.linalg.expm(-1j * LOE * (L / E) * Hright) @ U.conj().T
+    textbook = np.conj(U) @ scipy.linalg.expm(-1j * LOE * (L / E) * Hconj) @ U.T
+    assert np.abs(A - right).max() < 1e-11
+    assert np.abs(np.abs(A) ** 2 - np.abs(textbook) ** 2).max() > 1e-3
+    g = load_golden("prem12_io_dcp")
+    P = g["P"].reshape(-1, 3, 3)
+    assert np.abs(P - P.transpose(0, 2, 1)).max() > 1e-3   # a transposed index would be visible
