@@ -119,16 +119,87 @@ def traffic_from_profile(workload, precision):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def run_scan(args, sync):
+    """Batched scan: 1024 sets (seeded), each step = new hypotheses for every set + one
+    set-up launch + one persistent pair launch, replayed as a captured HIP graph."""
+    from cudaprob3_amd import metrics as M
+    from cudaprob3_amd import workloads as W
+    from cudaprob3_amd.batch import BatchPropagator, SetSpec
+
+    specs = [SetSpec(s.cosines, s.energies, s.radii, s.rhos, s.params, s.antinu, s.h_km) for s in W.scan_sets()]
+    b = BatchPropagator(specs, graph=False)
+    waves, gpc, tuned_ms = b.autotune()
+    log(f"[bench] scan1024: {b.n_pairs} pairs, autotuned {waves} waves/workgroup x {gpc} workgroups/CU "
+        f"({tuned_ms:.3f} ms pair kernel)")
+    b.compute()
+    total_ms, k_ms = b.last_ms()
+    b.use_graph(True)
+    rng = np.random.Generator(np.random.PCG64(33))
+    hyps = [np.array([W.random_params(rng) for _ in specs]) for _ in range(args.steps + args.warmup)]
+    for k in range(args.warmup):
+        b.set_params_many(hyps[k])
+        b.compute()
+    sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        b.set_params_many(hyps[args.warmup + k])
+        b.compute()
+    sync()
+    dt = time.perf_counter() - t0
+    value = args.steps * b.n_pairs / dt
+    log(f"[bench] {args.steps} scan steps in {dt:.4f} s -> {value:.4e} pairs/s")
+    flops = 0.0
+    for h in b.hosts:
+        flops += M.grid_flops(h.maxlayers, h.n_energies)
+    achieved = flops / (k_ms * 1e-3) / 1e12
+    out_bytes = 36 * b.n_pairs
+    result = {
+        "metric": "pairs/s: batched parameter scan, mixed FP32/FP64, one persistent launch",
+        "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (f64 eigenvalues and phase reduction)", "data": "synthetic",
+        "config": {"workload": "scan1024: 1024 parameter sets, cosZ 64-512 x E 64-1024 each, 4-50 shell "
+                               "random Earths, new hypotheses every step, HIP-graph replay",
+                   "pairs_per_step": b.n_pairs, "autotuned": {"waves_per_group": waves, "groups_per_cu": gpc}},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": M.PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / M.PEAK_FP32_VALU_TFLOPS, "traffic": traffic_from_profile("scan1024", "mixed"),
+                     "kernel": "batch_pairs", "kernel_ms": k_ms, "all_kernels_ms": total_ms,
+                     "algorithmic_flops_per_launch": flops, "output_GBps": out_bytes / (k_ms * 1e-3) / 1e9},
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        kind = "reference" if oracle.ref_available() else "port"
+        fn = oracle.ref_propagate if kind == "reference" else oracle.propagate
+        t0 = time.perf_counter()
+        pairs = 0
+        for s in b.sets[:64]:
+            fn(s.antinu, s.cosines, s.energies, s.radii, s.rhos, s.h_km, s.params, threads=threads)
+            pairs += len(s.cosines) * len(s.energies)
+        cdt = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": pairs / cdt, "unit": "pairs/s", "cores": threads, "kind": kind,
+                                  "sample": f"the first 64 of the 1024 sets ({pairs} pairs), {cdt:.1f} s"}
+        log(f"[bench] CPU baseline {pairs / cdt:.4e} pairs/s")
+    b.close()
+    return result
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="jitter12", choices=["jitter12", "scan1024"],
+                    help="jitter12: the 2048^2 grid, four variants per step; scan1024: 1024 parameter sets "
+                         "with uneven grids and Earth models in one persistent launch per step")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64", "fp64_plain"])
     ap.add_argument("--n", type=int, default=2048, help="grid edge (cosines = energies = n)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other precisions' throughput")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run one elementwise copy of this many MiB (reads and writes that many "
+                         "bytes), the known-size kernel that scales HBM counters read by a profiler")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -145,6 +216,18 @@ def main():
         sys.exit(1)
     torch.cuda.set_device(0)
     sync = torch.cuda.synchronize
+    if args.copy_calibration_mib:
+        src = torch.ones(args.copy_calibration_mib * (1 << 18), dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        sync()
+        torch.neg(src, out=dst)
+        sync()
+        log(f"[bench] calibration copy: {src.numel() * 4} bytes read, {dst.numel() * 4} bytes written")
+        del src, dst
+
+    if args.workload == "scan1024":
+        print(json.dumps(run_scan(args, sync)), flush=True)
+        return
 
     grids = W.jitter12_variants(args.n)
     workload = f"jitter12_{args.n}"

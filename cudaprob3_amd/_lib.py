@@ -47,6 +47,24 @@ SIGNATURES = {
     "cp3h_stream": (ctypes.c_int, [_handle, ctypes.POINTER(ctypes.c_void_p)]),
     "cp3h_last_compute_ms": (ctypes.c_int, [_handle, _c_float_p, _c_float_p]),
     "cp3h_last_error": (ctypes.c_char_p, []),
+    "cp3h_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p,
+                                         ctypes.POINTER(_handle)]),
+    "cp3h_batch_destroy": (ctypes.c_int, [_handle]),
+    "cp3h_batch_set_grid_f64": (ctypes.c_int, [_handle, ctypes.c_int, _c_double_p, _c_int_p, _c_double_p]),
+    "cp3h_batch_set_density_f64": (ctypes.c_int, [_handle, ctypes.c_int, _c_double_p, _c_double_p]),
+    "cp3h_batch_set_params_f64": (ctypes.c_int, [_handle, ctypes.c_int, _c_double_p, _c_double_p, ctypes.c_int,
+                                                 ctypes.c_double]),
+    "cp3h_batch_set_params_many_f64": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p,
+                                                      _c_int_p, _c_double_p]),
+    "cp3h_batch_config": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_int]),
+    "cp3h_batch_autotune": (ctypes.c_int, [_handle, _c_int_p, _c_int_p, _c_float_p]),
+    "cp3h_batch_use_graph": (ctypes.c_int, [_handle, ctypes.c_int]),
+    "cp3h_batch_compute_async": (ctypes.c_int, [_handle]),
+    "cp3h_batch_wait": (ctypes.c_int, [_handle]),
+    "cp3h_batch_compute": (ctypes.c_int, [_handle]),
+    "cp3h_batch_last_ms": (ctypes.c_int, [_handle, _c_float_p, _c_float_p]),
+    "cp3h_batch_copy_results_f32": (ctypes.c_int, [_handle, ctypes.c_int, _c_float_p, ctypes.c_size_t]),
+    "cp3h_batch_device_results": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
 }
 
 _LIB: ctypes.CDLL | None = None

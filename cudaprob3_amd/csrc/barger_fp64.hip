@@ -7,7 +7,7 @@
 //                            cosine row x 64 energies, the AXFAC table in LDS, parameters
 //                            by value in the kernel-argument segment.
 //  * layer_geometry       -- per-cosine layer lengths (physics.hpp:553-580, 615-618).
-//  * shell_spectrum<T>    -- per-(shell, energy) eigenvalues and flavour-basis projectors.
+//  * shell_spectrum_*     -- per-(shell, energy) eigenvalues and flavour-basis projectors.
 //                            A layer's amplitude is U (sum_k e^{i phi_k} P_k) U^dagger with
 //                            P_k the reference's eq. 11 products (physics.hpp:346-431); the
 //                            P_k depend on (E, rho) only, so they are computed once per
@@ -279,35 +279,29 @@ __global__ __launch_bounds__(kBlock) void propagate_plain_f64(GridDesc g, OscPar
         }
 }
 
+// Length in km of layer i of cosine row ic (0 beyond the row's crossed layers).
+__device__ inline double geometry_cell(const GridDesc& g, int ic, int i) {
+    const int L = g.maxlayers[ic];
+    if (i > L) return 0.0;
+    const double cz = g.cosines[ic];
+    const double path = path_length(cz, g.h_cm);
+    const double earth_path = -2.0 * cz * kREarthCm;
+    return layer_distance(g.radii, i, L, path, earth_path, cz) / kKm2Cm;
+}
+
 __global__ __launch_bounds__(kBlock) void layer_geometry(GridDesc g, double* __restrict__ geom) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int slots = g.rows + 1;
     if (t >= g.n_c * slots) return;
-    const int ic = t / slots, i = t % slots;
-    const int L = g.maxlayers[ic];
-    double v = 0.0;
-    if (i <= L) {
-        const double cz = g.cosines[ic];
-        const double path = path_length(cz, g.h_cm);
-        const double earth_path = -2.0 * cz * kREarthCm;
-        v = layer_distance(g.radii, i, L, path, earth_path, cz) / kKm2Cm;
-    }
-    geom[t] = v;
+    geom[t] = geometry_cell(g, t / slots, t % slots);
 }
 
-// Per (shell s, energy e): Q_k = U P_k U^dagger (Hermitian, packed to 9 reals per k) and the
-// phase coefficient of eigenvalue k.  kRevolutions: R = -2.534 lambda_k / (2 pi E), the
-// per-pair phase in turns is R * L; otherwise R = -2.534 lambda_k and the phase is R*L/E.
-template <typename T, bool kRevolutions>
-__global__ __launch_bounds__(kBlock) void shell_spectrum(GridDesc g, OscParams p, int ne_pad,
-                                                         T* __restrict__ Q, double* __restrict__ R) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (g.rows + 1) * ne_pad) return;
-    const int s = t / ne_pad, e = t % ne_pad;
-    const double E = g.energies[e < g.n_e ? e : g.n_e - 1];
-    const double rho = (s == 0 ? 0.0 : g.rhos[s - 1]) * kYe;
-
-    double mm[3][3], mv[3][3];
+// Flavour-basis projectors of one (energy, density) layer Hamiltonian:
+// Q_k = U P_k U^dagger (P_k the eq. 11 products), Hermitian, packed as 9 reals per k:
+// (Q00, Q11, Q22, Re Q01, Im Q01, Re Q02, Im Q02, Re Q12, Im Q12).  Also returns the
+// eigenvalue differences mm and mv of matter_masses.
+__device__ inline void flavour_projectors(const OscParams& p, double E, double rho, double Q[3][9],
+                                          double mm[3][3], double mv[3][3]) {
     matter_masses(p, E, rho, mm, mv);
     TwoEH h;
     two_eh(p, E, rho, mv, h);
@@ -320,14 +314,9 @@ __global__ __launch_bounds__(kBlock) void shell_spectrum(GridDesc g, OscParams p
             projector_elem(h, mm, i, j, v);
             P[0][i][j] = v[0]; P[1][i][j] = v[1]; P[2][i][j] = v[2];
         }
-    const std::size_t st = ne_pad;
-    T* q = Q + std::size_t(s) * 27 * st + e;
-    double* r = R + std::size_t(s) * 3 * st + e;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        // Q_k[n][m] for (n,m) in {00, 11, 22, 01, 02, 12}
         const int nm[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
-        double packed[9];
 #pragma unroll
         for (int w = 0; w < 6; w++) {
             const int n = nm[w][0], m = nm[w][1];
@@ -342,16 +331,111 @@ __global__ __launch_bounds__(kBlock) void shell_spectrum(GridDesc g, OscParams p
                     a.im += f[2] * P[k][i][j].im + f[3] * P[k][i][j].re;
                 }
             if (w < 3) {
-                packed[w] = a.re;
+                Q[k][w] = a.re;
             } else {
-                packed[3 + 2 * (w - 3)] = a.re;
-                packed[4 + 2 * (w - 3)] = a.im;
+                Q[k][3 + 2 * (w - 3)] = a.re;
+                Q[k][4 + 2 * (w - 3)] = a.im;
             }
         }
+    }
+}
+
+__device__ inline void slot_inputs(const GridDesc& g, int s, int e, double& E, double& rho) {
+    E = g.energies[e < g.n_e ? e : g.n_e - 1];
+    rho = (s == 0 ? 0.0 : g.rhos[s - 1]) * kYe;
+}
+
+// FP64 table, per (slot s, energy e): all three Q_k (Q[(s*27 + k*9 + f)*ne_pad + e]) and the
+// phase coefficients R[(s*3 + k)*ne_pad + e] = -2.534 lambda_k (the per-pair phase is R*L/E,
+// the reference's expression).
+__global__ __launch_bounds__(kBlock) void shell_spectrum_f64(GridDesc g, OscParams p, int ne_pad,
+                                                             double* __restrict__ Q, double* __restrict__ R) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (g.rows + 1) * ne_pad) return;
+    const int s = t / ne_pad, e = t % ne_pad;
+    double E, rho;
+    slot_inputs(g, s, e, E, rho);
+    double q[3][9], mm[3][3], mv[3][3];
+    flavour_projectors(p, E, rho, q, mm, mv);
+    const std::size_t st = ne_pad;
 #pragma unroll
-        for (int f = 0; f < 9; f++) q[(k * 9 + f) * st] = T(packed[f]);
-        const double c = -kLoE * mv[k][0];
-        r[k * st] = kRevolutions ? c / (E * (2.0 * kPi)) : c;
+    for (int k = 0; k < 3; k++) {
+#pragma unroll
+        for (int f = 0; f < 9; f++) Q[(std::size_t(s) * 27 + k * 9 + f) * st + e] = q[k][f];
+        R[(std::size_t(s) * 3 + k) * st + e] = -kLoE * mv[k][0];
+    }
+}
+
+// Mixed-precision table, per (slot s, energy e).  The layer amplitude up to a global phase is
+// I + (e^{i dphi_1} - 1) Q_1 + (e^{i dphi_2} - 1) Q_2, since sum_k Q_k = I, with
+// dphi_k = -2.534 (lambda_k - lambda_0) L / E.  Stored: Q_1 and Q_2 in float as five float4
+// groups Qm[((s*5 + g)*ne_pad + e)*4 + c] (packed index g*4 + c: 0-8 Q_1, 9-17 Q_2, 18-19
+// zero), and dphi_k / L in turns as Rm[(s*ne_pad + e)*2 + k - 1], double.
+__device__ inline void spectrum_cell_mixed(const GridDesc& g, const OscParams& p, int ne_pad, int s, int e,
+                                           float* __restrict__ Qm, double* __restrict__ Rm) {
+    double E, rho;
+    slot_inputs(g, s, e, E, rho);
+    double q[3][9], mm[3][3], mv[3][3];
+    flavour_projectors(p, E, rho, q, mm, mv);
+    float packed[20];
+#pragma unroll
+    for (int f = 0; f < 9; f++) {
+        packed[f] = float(q[1][f]);
+        packed[9 + f] = float(q[2][f]);
+    }
+    packed[18] = 0.0f;
+    packed[19] = 0.0f;
+    const std::size_t st = ne_pad;
+    float4* dst = reinterpret_cast<float4*>(Qm);
+#pragma unroll
+    for (int grp = 0; grp < 5; grp++)
+        dst[(std::size_t(s) * 5 + grp) * st + e] =
+            make_float4(packed[4 * grp], packed[4 * grp + 1], packed[4 * grp + 2], packed[4 * grp + 3]);
+    const double turn = 1.0 / (E * (2.0 * kPi));
+    Rm[(std::size_t(s) * st + e) * 2 + 0] = -kLoE * mm[1][0] * turn;
+    Rm[(std::size_t(s) * st + e) * 2 + 1] = -kLoE * mm[2][0] * turn;
+}
+
+__global__ __launch_bounds__(kBlock) void shell_spectrum_mixed(GridDesc g, OscParams p, int ne_pad,
+                                                               float* __restrict__ Qm, double* __restrict__ Rm) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (g.rows + 1) * ne_pad) return;
+    spectrum_cell_mixed(g, p, ne_pad, t / ne_pad, t % ne_pad, Qm, Rm);
+}
+
+// Set index of work cell t given the per-set prefix sums off[0..n] (off[0] = 0).
+__device__ inline int find_set(const long long* __restrict__ off, int n, long long t) {
+    int lo = 0, hi = n;          // off[lo] <= t < off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Batched set-up of many parameter sets in one launch: work cells [0, G) are layer lengths
+// (set, cosine, layer), [G, G + S) spectrum cells (set, slot, energy).
+__global__ __launch_bounds__(kBlock) void batch_setup(BatchDesc b) {
+    const long long G = b.geom_prefix[b.n_sets], S = b.spec_prefix[b.n_sets];
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < G + S;
+         t += (long long)gridDim.x * blockDim.x) {
+        if (t < G) {
+            const int k = find_set(b.geom_prefix, b.n_sets, t);
+            const BatchSet& set = b.sets[k];
+            const GridDesc g = batch_grid(b, set);
+            const long long c = t - b.geom_prefix[k];
+            const int slots = set.rows + 1;
+            b.geom[set.geom_off + c] = geometry_cell(g, int(c / slots), int(c % slots));
+        } else {
+            const long long u = t - G;
+            const int k = find_set(b.spec_prefix, b.n_sets, u);
+            const BatchSet& set = b.sets[k];
+            const GridDesc g = batch_grid(b, set);
+            const long long c = u - b.spec_prefix[k];
+            spectrum_cell_mixed(g, set.p, set.ne_pad, int(c / set.ne_pad), int(c % set.ne_pad),
+                                reinterpret_cast<float*>(b.Q + set.tab_off * 5),
+                                reinterpret_cast<double*>(b.R + set.tab_off));
+        }
     }
 }
 
@@ -372,21 +456,26 @@ hipError_t launch_geometry(const GridDesc& g, double* geom, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_spectrum_f32(const GridDesc& g, const OscParams& p, int ne_pad, float* Q, double* R,
-                               hipStream_t s) {
+hipError_t launch_spectrum_mixed(const GridDesc& g, const OscParams& p, int ne_pad, float* Qm, double* Rm,
+                                 hipStream_t s) {
     const unsigned n = unsigned(g.rows + 1) * ne_pad;
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL((shell_spectrum<float, true>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g, p,
-                       ne_pad, Q, R);
+    if (n == 0 || g.n_e == 0) return hipSuccess;
+    hipLaunchKernelGGL(shell_spectrum_mixed, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g, p, ne_pad,
+                       Qm, Rm);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_setup(const BatchDesc& b, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(batch_setup, dim3(blocks), dim3(kBlock), 0, s, b);
     return hipGetLastError();
 }
 
 hipError_t launch_spectrum_f64(const GridDesc& g, const OscParams& p, int ne_pad, double* Q, double* R,
                                hipStream_t s) {
     const unsigned n = unsigned(g.rows + 1) * ne_pad;
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL((shell_spectrum<double, false>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g,
-                       p, ne_pad, Q, R);
+    if (n == 0 || g.n_e == 0) return hipSuccess;
+    hipLaunchKernelGGL(shell_spectrum_f64, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g, p, ne_pad, Q,
+                       R);
     return hipGetLastError();
 }
 

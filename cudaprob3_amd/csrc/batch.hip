@@ -1,0 +1,68 @@
+// Persistent pair kernel of a batched parameter scan: many parameter sets, each with its
+// own uneven grid and Earth model, propagated in one launch.
+//
+// Work items are (set, 64-energy tile, cosine row), numbered set-major, then tile-major, so
+// consecutive items share a spectrum-table slice.  A fixed grid of workgroups (sized from
+// the CU count by the autotuner) walks the item list: wavefront w of the logical grid takes
+// items w, w + W, w + 2W, ...; logical workgroup ids are laid out so that the wavefronts of
+// one XCD take consecutive items (same table slice in that XCD's L2).  Each wavefront is
+// independent: no barriers, and every wavefront leaves the loop once its next item is past
+// the end of the list.
+#include "common.hpp"
+#include "pair_mixed.hpp"
+
+namespace cp3 {
+namespace {
+
+__device__ inline int item_set(const long long* __restrict__ off, int n, long long t) {
+    int lo = 0, hi = n;          // off[lo] <= t < off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <int kWaves>
+__global__ __launch_bounds__(kWaves * kWave) void batch_pairs(BatchDesc b) {
+    const long long total = b.item_prefix[b.n_sets];
+    const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / kWave);
+    const int lane = int(threadIdx.x) % kWave;
+    const long long W = (long long)gridDim.x * kWaves;
+    for (long long item = (long long)lb * kWaves + wave; item < total; item += W) {
+        const int k = __builtin_amdgcn_readfirstlane(item_set(b.item_prefix, b.n_sets, item));
+        const BatchSet& set = b.sets[k];
+        const long long local = item - b.item_prefix[k];
+        const int tile = int(local / set.n_c);
+        const int ic = int(local % set.n_c);
+        const int ie = tile * kWave + lane;            // < ne_pad
+        const int L = b.maxlayers[set.cos_off + ic];
+        const double* gl = b.geom + set.geom_off + std::size_t(ic) * (set.rows + 1);
+        float p[3][3];
+        propagate_pair_mixed(L, gl, b.Q + set.tab_off * 5, b.R + set.tab_off, set.ne_pad, ie, p);
+        if (ie < set.n_e) {
+            const std::size_t plane = std::size_t(set.n_c) * set.n_e;
+            float* o = b.out + set.out_off + std::size_t(ic) * set.n_e + ie;
+#pragma unroll
+            for (int in = 0; in < 3; in++)
+#pragma unroll
+                for (int ou = 0; ou < 3; ou++) o[(in * 3 + ou) * plane] = p[in][ou];
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_batch_pairs(const BatchDesc& b, int blocks, int waves, hipStream_t s) {
+    if (blocks <= 0 || blocks % kXcds != 0) return hipErrorInvalidValue;
+    switch (waves) {
+        case 4: hipLaunchKernelGGL(batch_pairs<4>, dim3(blocks), dim3(4 * kWave), 0, s, b); break;
+        case 8: hipLaunchKernelGGL(batch_pairs<8>, dim3(blocks), dim3(8 * kWave), 0, s, b); break;
+        case 16: hipLaunchKernelGGL(batch_pairs<16>, dim3(blocks), dim3(16 * kWave), 0, s, b); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace cp3

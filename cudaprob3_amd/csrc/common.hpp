@@ -60,12 +60,15 @@ hipError_t launch_plain_f64(const GridDesc& g, const OscParams& p, double* out, 
 // Per-cosine layer lengths in km: geom[ic*(rows+1) + i], i = 0..maxlayers[ic].
 hipError_t launch_geometry(const GridDesc& g, double* geom, hipStream_t s);
 
-// Per-(shell, energy) spectrum: flavour-basis projectors and phase coefficients.
-// Layout: Q[(s*27 + f)*ne_pad + e], R[(s*3 + k)*ne_pad + e], s = 0..rows (s = 0 is vacuum,
-// s >= 1 has the density rhos[s-1]).
-// Q is float or double; R holds -2.534*lambda_k (double table) or -2.534*lambda_k/(2 pi E)
-// (float table: phase in revolutions).
-hipError_t launch_spectrum_f32(const GridDesc& g, const OscParams& p, int ne_pad, float* Q, double* R, hipStream_t s);
+// Per-(shell, energy) spectrum tables, s = 0..rows (s = 0 is vacuum, s >= 1 has the density
+// rhos[s-1]); ne_pad = n_e rounded up to 64.
+//  FP64:  Q[(s*27 + k*9 + f)*ne_pad + e] (Q_k packed, 9 reals), R[(s*3 + k)*ne_pad + e] =
+//         -2.534 lambda_k (phase R*L/E).
+//  mixed: Qm float4 groups [(s*5 + g)*ne_pad + e] holding Q_1, Q_2 (18 floats + 2 zero),
+//         Rm[(s*ne_pad + e)*2 + k-1] = -2.534 (lambda_k - lambda_0) / (2 pi E) (turns per km).
+constexpr int kMixedQFloats = 20;
+constexpr int kMixedRDoubles = 2;
+hipError_t launch_spectrum_mixed(const GridDesc& g, const OscParams& p, int ne_pad, float* Qm, double* Rm, hipStream_t s);
 hipError_t launch_spectrum_f64(const GridDesc& g, const OscParams& p, int ne_pad, double* Q, double* R, hipStream_t s);
 
 // Table-driven per-pair kernels.
@@ -73,5 +76,49 @@ hipError_t launch_pairs_mixed(const GridDesc& g, int ne_pad, const double* geom,
                               const double* R, float* out, hipStream_t s);
 hipError_t launch_pairs_f64(const GridDesc& g, int ne_pad, const double* geom, const double* Q,
                             const double* R, double* out, hipStream_t s);
+
+// ---- batched parameter scans ------------------------------------------------------------
+
+// One parameter set of a batch: its hypothesis and where its grid, Earth model, layer
+// lengths, spectrum table and results sit in the batch's concatenated device arrays.
+struct BatchSet {
+    OscParams p;
+    double h_cm;
+    long long cos_off;    // cosines / maxlayers
+    long long e_off;      // energies
+    long long model_off;  // radii / rhos (descending)
+    long long geom_off;   // layer lengths, n_c * (rows + 1)
+    long long tab_off;    // spectrum cells, (rows + 1) * ne_pad
+    long long out_off;    // results, 9 * n_c * n_e floats (SoA per set)
+    int n_c, n_e, rows, ne_pad;
+};
+
+struct BatchDesc {
+    const BatchSet* sets;
+    int n_sets;
+    const long long* geom_prefix;   // [n_sets + 1] prefix sums of n_c * (rows + 1)
+    const long long* spec_prefix;   // [n_sets + 1] prefix sums of (rows + 1) * ne_pad
+    const long long* item_prefix;   // [n_sets + 1] prefix sums of n_c * ceil(n_e / 64)
+    const double* cosines;
+    const int* maxlayers;
+    const double* energies;
+    const double* radii;
+    const double* rhos;
+    double* geom;
+    float4* Q;
+    double2* R;
+    float* out;
+};
+
+__device__ inline GridDesc batch_grid(const BatchDesc& b, const BatchSet& s) {
+    return GridDesc{b.cosines + s.cos_off, b.energies + s.e_off, b.maxlayers + s.cos_off,
+                    b.radii + s.model_off, b.rhos + s.model_off, s.n_c, s.n_e, s.rows, s.h_cm};
+}
+
+// Set-up of every set (layer lengths + mixed spectrum tables), grid-stride over all cells.
+hipError_t launch_batch_setup(const BatchDesc& b, int blocks, hipStream_t s);
+// Persistent pair kernel over all sets: `blocks` workgroups of `waves` wavefronts
+// (waves in {4, 8, 16}); each wavefront takes (set, cosine row, 64-energy tile) items.
+hipError_t launch_batch_pairs(const BatchDesc& b, int blocks, int waves, hipStream_t s);
 
 }  // namespace cp3

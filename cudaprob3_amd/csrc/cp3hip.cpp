@@ -7,6 +7,7 @@
 // kernels by value at launch, so there is no process-global device state.
 #include "../../include/cp3hip.h"
 #include "common.hpp"
+#include "host_util.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -16,52 +17,8 @@
 #include <string>
 #include <vector>
 
-namespace {
-
-thread_local std::string g_error;
-
-int fail(int code, const std::string& msg) {
-    g_error = msg;
-    return code;
-}
-
-#define CP3H_HIP(call)                                                                   \
-    do {                                                                                 \
-        hipError_t err_ = (call);                                                        \
-        if (err_ != hipSuccess)                                                          \
-            return fail(CP3H_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(err_)); \
-    } while (0)
-
-// Vacuum cubic roots matched to DM(i,0) (physics.hpp:198-253), in the caller's precision.
-template <typename T>
-void vacuum_order(const T* DM, int order[3]) {
-    const T a = DM[1] + DM[2];
-    const T b = DM[1] * DM[2];
-    const T g = 0.0;
-    T t = a * a - T(3.0) * b;
-    T arg = (T(2.0) * a * a * a - T(9.0) * a * b + T(27.0) * g) / (T(2.0) * std::sqrt(t * t * t));
-    if (std::fabs(arg) > T(1.0)) arg = arg / std::fabs(arg);
-    const T th0 = std::acos(arg) / T(3.0);
-    const T th[3] = {th0, T(th0 - (2.0 * M_PI / 3.0)), T(th0 + (2.0 * M_PI / 3.0))};
-    T root[3];
-    for (int k = 0; k < 3; k++) {
-        root[k] = T(-(2.0 / 3.0) * std::sqrt(t));
-        root[k] *= std::cos(th[k]);
-    }
-    t = DM[0] - a / T(3.0);
-    for (int k = 0; k < 3; k++) root[k] += t;
-    for (int i = 0; i < 3; i++) {
-        T best = std::fabs(DM[i * 3] - root[0]);
-        int k = 0;
-        for (int j = 1; j < 3; j++) {
-            const T d = std::fabs(DM[i * 3] - root[j]);
-            if (d < best) { k = j; best = d; }
-        }
-        order[i] = k;
-    }
-}
-
-}  // namespace
+using cp3host::fail;
+using cp3host::vacuum_order;
 
 struct cp3h_propagator {
     int device = 0, n_c = 0, n_e = 0, precision = CP3H_FP64_PLAIN, ne_pad = 0, rows = 0;
@@ -82,7 +39,6 @@ struct cp3h_propagator {
     bool has_energies = false, has_cosines = false, has_density = false, has_params = false;
     bool has_timing = false;
 
-    int elem_q() const { return precision == CP3H_MIXED ? 4 : 8; }
     std::size_t n_results() const { return std::size_t(9) * n_c * n_e; }
 };
 
@@ -116,7 +72,7 @@ int upload(cp3h_propagator* h, void* dst, const void* src, std::size_t bytes) {
 
 extern "C" {
 
-const char* cp3h_last_error(void) { return g_error.c_str(); }
+const char* cp3h_last_error(void) { return cp3host::g_error.c_str(); }
 
 int cp3h_device_count(int* count) {
     if (!count) return fail(CP3H_ERR_ARGUMENT, "null count");
@@ -260,8 +216,9 @@ int cp3h_set_density_f64(cp3h_propagator* h, const double* radii_desc, const dou
         CP3H_HIP(hipMalloc(&h->d_geom, sizeof(double) * std::max<std::size_t>(std::size_t(h->n_c) * slots, 1)));
         if (h->precision != CP3H_FP64_PLAIN) {
             const std::size_t cells = slots * h->ne_pad;
-            CP3H_HIP(hipMalloc(&h->d_Q, cells * 27 * h->elem_q()));
-            CP3H_HIP(hipMalloc(&h->d_R, cells * 3 * sizeof(double)));
+            const bool mixed = h->precision == CP3H_MIXED;
+            CP3H_HIP(hipMalloc(&h->d_Q, cells * (mixed ? cp3::kMixedQFloats * sizeof(float) : 27 * sizeof(double))));
+            CP3H_HIP(hipMalloc(&h->d_R, cells * (mixed ? cp3::kMixedRDoubles : 3) * sizeof(double)));
         }
         h->rows = n_rows;
     }
@@ -323,7 +280,7 @@ int cp3h_compute_async(cp3h_propagator* h, int antineutrino, double production_h
     } else {
         CP3H_HIP(cp3::launch_geometry(g, h->d_geom, h->stream));
         if (h->precision == CP3H_MIXED) {
-            CP3H_HIP(cp3::launch_spectrum_f32(g, h->params, h->ne_pad, static_cast<float*>(h->d_Q), h->d_R, h->stream));
+            CP3H_HIP(cp3::launch_spectrum_mixed(g, h->params, h->ne_pad, static_cast<float*>(h->d_Q), h->d_R, h->stream));
             CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
             CP3H_HIP(cp3::launch_pairs_mixed(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_R,
                                              static_cast<float*>(h->d_out), h->stream));

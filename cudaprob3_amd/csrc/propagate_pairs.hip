@@ -7,22 +7,32 @@
 // phi_k = -2.534 lambda_k L / E, the sum over k, and the running products
 // (physics.hpp:645-677) remain.
 //
-// Mixed precision (pairs_mixed): the phase is formed and reduced modulo one turn in FP64
-// (it reaches ~10^3 rad on core-crossing baselines at 0.05 GeV, where FP32 would lose
-// 1e-4 rad), then the hardware sin/cos (v_sin_f32/v_cos_f32, argument in turns) and all
-// matrix arithmetic run in FP32.  FP64 (pairs_f64): the same structure in double with the
-// reference's phase expression ((-2.534 lambda_k) L) / E.
+// FP64 (pairs_f64): A = sum_k (cos phi_k + i sin phi_k) Q_k with the reference's phase
+// expression ((-2.534 lambda_k) L) / E, and the reference's running products of full 3x3
+// matrices.
+//
+// Mixed precision (pairs_mixed): three changes that are exact in real arithmetic.
+//  * The layer's global phase e^{i phi_0} is dropped (it cancels in |amplitude|^2), and
+//    sum_k Q_k = I gives A' = I + (e^{i dphi_1} - 1) Q_1 + (e^{i dphi_2} - 1) Q_2 with
+//    dphi_k = phi_k - phi_0.  dphi_k (up to ~10^3 rad on core-crossing paths at 0.05 GeV,
+//    where FP32 would lose 1e-4 rad) is formed and reduced modulo one turn in FP64; the
+//    hardware sin/cos (v_sin_f32 / v_cos_f32, argument in turns) and the matrix arithmetic
+//    are FP32.
+//  * The total amplitude is unitary, so its upper-left 2x2 block fixes all nine
+//    probabilities (rows and columns of |S|^2 sum to one).  Only columns 0-1 of the forward
+//    product and rows 0-1 of the mantle product are carried: 18 + 18 complex MACs per
+//    mantle layer instead of 27 + 27.
+//  * Q_1, Q_2 are read as five coalesced float4 loads per lane and layer.
 //
 // Mapping: one wavefront = one cosine row x 64 consecutive energies (lane = energy); four
 // wavefronts of a workgroup take four consecutive rows of the same energy tile, and
 // consecutive logical workgroups stay on one XCD so that an energy tile's table slice is
 // reused from that XCD's L2.  Layer lengths of the row are wave-uniform scalar loads.
 #include "common.hpp"
+#include "pair_mixed.hpp"
 
 namespace cp3 {
 namespace {
-
-template <typename T> struct Cx { T re, im; };
 
 template <typename T>
 __device__ inline void mul3(const Cx<T> A[3][3], const Cx<T> B[3][3], Cx<T> C[3][3]) {
@@ -82,9 +92,9 @@ struct PairArgs {
     int n_c, n_e, rows, ne_pad;
 };
 
-// kMixed: T = float, R in turns; otherwise T = double, R = -2.534 lambda.
-template <typename T, bool kMixed>
-__global__ __launch_bounds__(kBlock) void pairs_kernel(PairArgs a) {
+// FP64 pair kernel: R = -2.534 lambda_k, phase = R * L / E.
+__global__ __launch_bounds__(kBlock) void pairs_f64_kernel(PairArgs a) {
+    using T = double;
     const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
     const int tiles = (a.n_e + kWave - 1) / kWave;
     const int row_groups = (a.n_c + 3) / 4;
@@ -97,8 +107,7 @@ __global__ __launch_bounds__(kBlock) void pairs_kernel(PairArgs a) {
     const double* gl = a.geom + std::size_t(ic) * (a.rows + 1);
     const T* Q = static_cast<const T*>(a.Q);
     const std::size_t st = a.ne_pad;
-    double E = 0.0;
-    if (!kMixed) E = a.energies[ie < a.n_e ? ie : a.n_e - 1];
+    const double E = a.energies[ie < a.n_e ? ie : a.n_e - 1];
 
     Cx<T> fin[3][3], mantle[3][3], A[3][3], tmp[3][3];
     for (int i = 0; i <= L; i++) {
@@ -111,19 +120,8 @@ __global__ __launch_bounds__(kBlock) void pairs_kernel(PairArgs a) {
         T c[3], s[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            if (kMixed) {
-                double turns = r[k * st] * dist;
-                turns -= rint(turns);
-                const float x = float(turns);
-                s[k] = __builtin_amdgcn_sinf(x);
-                c[k] = __builtin_amdgcn_cosf(x);
-            } else {
-                const double arg = r[k * st] * dist / E;
-                double sv, cv;
-                sincos(arg, &sv, &cv);
-                s[k] = T(sv);
-                c[k] = T(cv);
-            }
+            const double arg = r[k * st] * dist / E;
+            sincos(arg, &s[k], &c[k]);
         }
         assemble<T>(qv, c, s, A);
         if (i == 0) {
@@ -172,14 +170,47 @@ __global__ __launch_bounds__(kBlock) void pairs_kernel(PairArgs a) {
         }
 }
 
-template <typename T, bool kMixed>
-hipError_t launch_pairs(const GridDesc& g, int ne_pad, const double* geom, const void* Q, const double* R,
-                        void* out, hipStream_t s) {
+// Mixed-precision pair kernel (see the file comment).
+struct MixedArgs {
+    const int* maxlayers;
+    const double* geom;       // [n_c][rows + 1] km
+    const float4* Q;          // [(rows + 1) * 5][ne_pad]
+    const double2* R;         // [(rows + 1)][ne_pad]
+    float* out;               // SoA [9][n_c][n_e]
+    int n_c, n_e, rows, ne_pad;
+};
+
+__global__ __launch_bounds__(kBlock) void pairs_mixed_kernel(MixedArgs a) {
+    const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int tiles = (a.n_e + kWave - 1) / kWave;
+    const int row_groups = (a.n_c + 3) / 4;
+    const int tile = lb / row_groups;
+    const int ic = (lb % row_groups) * 4 + threadIdx.x / kWave;
+    if (tile >= tiles || ic >= a.n_c) return;
+    const int ie = tile * kWave + threadIdx.x % kWave;   // < ne_pad: table reads stay in bounds
+
+    const int L = a.maxlayers[ic];
+    const double* gl = a.geom + std::size_t(ic) * (a.rows + 1);
+    const std::size_t st = a.ne_pad;
+
+    float p[3][3];
+    propagate_pair_mixed(L, gl, a.Q, a.R, st, ie, p);
+    if (ie >= a.n_e) return;
+    const std::size_t plane = std::size_t(a.n_c) * a.n_e;
+    float* o = a.out + std::size_t(ic) * a.n_e + ie;
+#pragma unroll
+    for (int in = 0; in < 3; in++)
+#pragma unroll
+        for (int ou = 0; ou < 3; ou++) o[(in * 3 + ou) * plane] = p[in][ou];
+}
+
+hipError_t launch_pairs_fp64(const GridDesc& g, int ne_pad, const double* geom, const void* Q, const double* R,
+                             void* out, hipStream_t s) {
     if (g.n_c == 0 || g.n_e == 0) return hipSuccess;
     PairArgs a{g.maxlayers, g.energies, geom, Q, R, out, g.n_c, g.n_e, g.rows, ne_pad};
     const unsigned tiles = (g.n_e + kWave - 1) / kWave;
     const unsigned blocks = round_up(tiles * ((g.n_c + 3) / 4), kXcds);
-    hipLaunchKernelGGL((pairs_kernel<T, kMixed>), dim3(blocks), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(pairs_f64_kernel, dim3(blocks), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 
@@ -187,12 +218,18 @@ hipError_t launch_pairs(const GridDesc& g, int ne_pad, const double* geom, const
 
 hipError_t launch_pairs_mixed(const GridDesc& g, int ne_pad, const double* geom, const float* Q,
                               const double* R, float* out, hipStream_t s) {
-    return launch_pairs<float, true>(g, ne_pad, geom, Q, R, out, s);
+    if (g.n_c == 0 || g.n_e == 0) return hipSuccess;
+    MixedArgs a{g.maxlayers, geom, reinterpret_cast<const float4*>(Q), reinterpret_cast<const double2*>(R), out,
+                g.n_c, g.n_e, g.rows, ne_pad};
+    const unsigned tiles = (g.n_e + kWave - 1) / kWave;
+    const unsigned blocks = round_up(tiles * ((g.n_c + 3) / 4), kXcds);
+    hipLaunchKernelGGL(pairs_mixed_kernel, dim3(blocks), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_pairs_f64(const GridDesc& g, int ne_pad, const double* geom, const double* Q,
                             const double* R, double* out, hipStream_t s) {
-    return launch_pairs<double, false>(g, ne_pad, geom, Q, R, out, s);
+    return launch_pairs_fp64(g, ne_pad, geom, Q, R, out, s);
 }
 
 }  // namespace cp3

@@ -110,6 +110,56 @@ int cp3h_last_compute_ms(cp3h_propagator* h, float* total_ms, float* pairs_ms);
 /* Message of the last failing call on this thread ("" if none). */
 const char* cp3h_last_error(void);
 
+/* ---- Batched parameter scans ------------------------------------------------------------
+ * No counterpart in the reference, whose propagators hold one hypothesis, one grid and one
+ * Earth model each: here many parameter sets, each with its own grid and Earth model, are
+ * propagated in one persistent launch (mixed precision, float results), with the set-up
+ * kernel and the pair kernel optionally replayed as a captured HIP graph.  Each set's inputs
+ * follow the single-propagator conventions above (host set-up by Propagator<FLOAT_T>). */
+typedef struct cp3h_batch cp3h_batch;
+
+/* n_sets sets; set i has n_cosines[i] x n_energies[i] pairs and an Earth model of n_rows[i]
+   rows.  All device memory is allocated here. */
+int cp3h_batch_create(int device, int n_sets, const int* n_cosines, const int* n_energies, const int* n_rows,
+                      cp3h_batch** out);
+int cp3h_batch_destroy(cp3h_batch* b);
+
+/* Grid of set i: cosines with their maxlayers, energies (GeV). */
+int cp3h_batch_set_grid_f64(cp3h_batch* b, int set, const double* cosines, const int* maxlayers,
+                            const double* energies);
+/* Earth model of set i, descending radii as for cp3h_set_density_f64. */
+int cp3h_batch_set_density_f64(cp3h_batch* b, int set, const double* radii_desc, const double* rhos_desc);
+/* Hypothesis of set i: U (18), DM (9), neutrino type, production height in cm.  May be
+   changed between computations (a parameter-scan step). */
+int cp3h_batch_set_params_f64(cp3h_batch* b, int set, const double* U18, const double* DM9, int antineutrino,
+                              double production_height_cm);
+/* The same for sets first .. first+count-1 in one call: U (18 per set), DM (9 per set),
+   neutrino types and heights (one per set). */
+int cp3h_batch_set_params_many_f64(cp3h_batch* b, int first, int count, const double* U18s, const double* DM9s,
+                                   const int* antineutrino, const double* production_height_cm);
+
+/* Launch configuration of the persistent pair kernel: waves per workgroup (4, 8, 16) and
+   workgroups per CU (1-16).  cp3h_batch_autotune times every combination on the current
+   inputs and keeps the fastest; returns them and the best time of the pair kernel. */
+int cp3h_batch_config(cp3h_batch* b, int waves_per_group, int groups_per_cu);
+int cp3h_batch_autotune(cp3h_batch* b, int* waves_per_group, int* groups_per_cu, float* best_ms);
+
+/* 1: computations replay a HIP graph (parameter upload, set-up kernel, pair kernel)
+   captured on first use and re-captured after a configuration change; 0: plain launches. */
+int cp3h_batch_use_graph(cp3h_batch* b, int enable);
+
+int cp3h_batch_compute_async(cp3h_batch* b);
+int cp3h_batch_wait(cp3h_batch* b);
+int cp3h_batch_compute(cp3h_batch* b);
+
+/* GPU time of the last computation (waits for it): all kernels, and the pair kernel alone
+   (pairs_ms is -1 when the computation was a graph replay). */
+int cp3h_batch_last_ms(cp3h_batch* b, float* total_ms, float* pairs_ms);
+
+/* Results of set i, SoA [t][ic][ie] float, count = 9 * n_cosines[i] * n_energies[i]. */
+int cp3h_batch_copy_results_f32(cp3h_batch* b, int set, float* dst, size_t count);
+int cp3h_batch_device_results(cp3h_batch* b, int set, const float** ptr);
+
 #ifdef __cplusplus
 }
 #endif

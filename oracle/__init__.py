@@ -82,6 +82,24 @@ def propagate(antinu, cosines, energies, radii, rhos, h_km, params, threads=0):
     return _propagate(oracle_lib().orc_propagate, antinu, cosines, energies, radii, rhos, h_km, params, threads)
 
 
+JITTER_SO = os.path.join(HERE, "liboracle_jitter.so")
+
+
+def libm_sensitivity(antinu, cosines, energies, radii, rhos, h_km, params, threads=0, reference=None):
+    """Largest |dP| by which the reference algorithm's FP64 result moves when its per-pair
+    sin/cos/acos results are each moved by up to one ulp (another faithfully rounded libm).
+    This is the reference's own rounding noise floor for a parity comparison."""
+    if reference is None:
+        reference = propagate(antinu, cosines, energies, radii, rhos, h_km, params, threads)
+    moved = _propagate(_load(JITTER_SO).orc_propagate, antinu, cosines, energies, radii, rhos, h_km, params, threads)
+    return float(np.abs(moved - reference).max())
+
+
+def fp64_tolerance(sensitivity: float, floor: float = 1e-12) -> float:
+    """FP64 parity bar: 1e-12, or twice the reference's own libm noise where that is larger."""
+    return max(floor, 2.0 * sensitivity)
+
+
 def ref_propagate(antinu, cosines, energies, radii, rhos, h_km, params, threads=1):
     """The compiled reference CpuPropagator<double>.  Same layout as :func:`propagate`."""
     return _propagate(ref_lib().ref_propagate, antinu, cosines, energies, radii, rhos, h_km, params, threads)

@@ -34,6 +34,31 @@ constexpr double kLoE = 2.534;         // physics.hpp:442   LoEfac
 
 struct Cx { double re, im; };
 
+// libm calls of the per-pair physics.  Built with -DCP3_ORACLE_LIBM_JITTER (liboracle_jitter.so)
+// each result is moved by one ulp up, one ulp down or left alone, chosen by a hash of the
+// argument: a stand-in for another faithfully rounded libm (such as the GPU's).  How far
+// that moves the probabilities is the reference algorithm's own sensitivity to libm
+// rounding, which sets the floor of any FP64 parity bar.
+#ifdef CP3_ORACLE_LIBM_JITTER
+inline double libm_jitter(double x, double y, std::uint64_t salt) {
+    std::uint64_t b;
+    std::memcpy(&b, &x, 8);
+    b ^= salt;
+    b ^= b >> 33;
+    b *= 0xff51afd7ed558ccdULL;
+    b ^= b >> 33;
+    const int r = int(b % 3);
+    return r == 0 ? y : std::nextafter(y, r == 1 ? 1e300 : -1e300);
+}
+inline double phys_sin(double x) { return libm_jitter(x, std::sin(x), 0x9e3779b97f4a7c15ULL); }
+inline double phys_cos(double x) { return libm_jitter(x, std::cos(x), 0xc2b2ae3d27d4eb4fULL); }
+inline double phys_acos(double x) { return libm_jitter(x, std::acos(x), 0x165667b19e3779f9ULL); }
+#else
+inline double phys_sin(double x) { return std::sin(x); }
+inline double phys_cos(double x) { return std::cos(x); }
+inline double phys_acos(double x) { return std::acos(x); }
+#endif
+
 // Parameter block shared by the physics routines (the reference keeps these in
 // process globals, physics.hpp:121-124; here they travel by reference).
 struct Params {
@@ -138,16 +163,16 @@ void matter_masses(const Params& p, double E, double rho, int antinu,
     const double argtmp = (2.0 * alpha * alpha * alpha - 9.0 * alpha * beta + 27.0 * gamma) /
                           (2.0 * std::sqrt(tmp * tmp * tmp));
     const double arg = std::fabs(argtmp) > 1.0 ? argtmp / std::fabs(argtmp) : argtmp;
-    const double th0 = std::acos(arg) / 3.0;
+    const double th0 = phys_acos(arg) / 3.0;
     const double th1 = th0 - (2.0 * M_PI / 3.0);
     const double th2 = th0 + (2.0 * M_PI / 3.0);
     double r[3];
     r[0] = -(2.0 / 3.0) * std::sqrt(tmp);
     r[1] = -(2.0 / 3.0) * std::sqrt(tmp);
     r[2] = -(2.0 / 3.0) * std::sqrt(tmp);
-    r[0] *= std::cos(th0);
-    r[1] *= std::cos(th1);
-    r[2] *= std::cos(th2);
+    r[0] *= phys_cos(th0);
+    r[1] *= phys_cos(th1);
+    r[2] *= phys_cos(th2);
     const double shift = D[0] - alpha / 3.0;
     r[0] += shift; r[1] += shift; r[2] += shift;
     double m[3];
@@ -209,8 +234,8 @@ void layer_amplitude(const Params& p, double L, double E, double rho, int antinu
         for (int j = 0; j < 3; j++) X[i][j] = {0.0, 0.0};
     for (int k = 0; k < 3; k++) {
         const double arg = -kLoE * mv[k][0] * L / E;
-        const double s = std::sin(arg);
-        const double c = std::cos(arg);
+        const double s = phys_sin(arg);
+        const double c = phys_cos(arg);
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) {
                 X[i][j].re += c * prod[i][j][k].re - s * prod[i][j][k].im;

@@ -189,3 +189,19 @@ def test_planted_mistakes_are_caught(oracle_mod):
     g = load_golden("prem12_io_dcp")
     P = g["P"].reshape(-1, 3, 3)
     assert np.abs(P - P.transpose(0, 2, 1)).max() > 1e-3   # a transposed index would be visible
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_libm_noise_floor_of_fixtures(oracle_mod, name):
+    """The FP64 parity bar is 1e-12 except where the reference's own result moves by more
+    than half of that under a one-ulp libm change; only the dm23 = 0 degeneracy nudge does,
+    and even there the bar stays below 1e-11."""
+    g = load_golden(name)
+    s = oracle_mod.libm_sensitivity(int(g["antinu"]), g["cosines"], g["energies"], g["radii"], g["rhos"],
+                                    float(g["h_km"]), g["params"], threads=2, reference=g["P"])
+    tol = oracle_mod.fp64_tolerance(s)
+    assert s > 0.0                       # the perturbation is live
+    if name == "degenerate_dm23_zero":
+        assert 1e-12 < tol < 1e-11
+    else:
+        assert tol == 1e-12
