@@ -30,69 +30,63 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_props(grids, precision):
+def make_prop(g, precision):
+    """A propagator holding one variant's grid, Earth model and hypothesis."""
     from cudaprob3_amd.propagator import HipPropagatorSingle
-    g0 = grids[0]
-    p = HipPropagatorSingle(0, len(g0.cosines), len(g0.energies), precision=precision)
-    p.setEnergyList(g0.energies)
-    p.setCosineList(g0.cosines)
-    p.setDensity(g0.radii, g0.rhos)
-    p.setProductionHeight(g0.h_km)
+    p = HipPropagatorSingle(0, len(g.cosines), len(g.energies), precision=precision)
+    p.setEnergyList(g.energies)
+    p.setCosineList(g.cosines)
+    p.setDensity(g.radii, g.rhos)
+    p.setProductionHeight(g.h_km)
+    p.setMNSMatrix(*g.params[:4])
+    p.setNeutrinoMasses(*g.params[4:])
     return p
 
 
-def run_variant(p, g, wait=False):
-    p.setMNSMatrix(*g.params[:4])
-    p.setNeutrinoMasses(*g.params[4:])
-    p.calculateProbabilitiesAsync(g.antinu)
-    if wait:
-        p.waitForCompletion()
-
-
-def time_steps(p, grids, steps, warmup, sync):
+def time_steps(props, grids, steps, warmup, sync):
+    """One step = every variant launched (in order, on the propagators' stream), then waited for."""
+    def step():
+        for p, g in zip(props, grids):
+            p.calculateProbabilitiesAsync(g.antinu)
+        for p in props:
+            p.waitForCompletion()
     for _ in range(warmup):
-        for g in grids:
-            run_variant(p, g)
-        p.waitForCompletion()
+        step()
     sync()
     t0 = time.perf_counter()
     for _ in range(steps):
-        for g in grids:
-            run_variant(p, g)
-        p.waitForCompletion()
+        step()
     sync()
     return time.perf_counter() - t0
 
 
-def kernel_ms(p, grids, reps):
-    """Average GPU duration of the per-pair kernel and of all kernels, per launch."""
+def kernel_ms(props, grids, reps):
+    """Average GPU duration of the per-pair kernel and of all kernels, per launch, with one
+    launch in flight at a time."""
     pairs, total = [], []
     for _ in range(reps):
-        for g in grids:
-            run_variant(p, g, wait=True)
+        for p, g in zip(props, grids):
+            p.calculateProbabilities(g.antinu)
             t, k = p.last_compute_times()
             total.append(t)
             pairs.append(k)
     return float(np.mean(pairs)), float(np.mean(total))
 
 
-def accuracy(grids, p_mixed):
+def accuracy(grids, props):
     """max / rms |dP| of the mixed kernel against the FP64 kernel (pinned to the reference
     within 1e-12 by the tests) over the full grids."""
-    p64 = make_props(grids, "fp64")
     worst, sq, n = 0.0, 0.0, 0
-    for g in grids:
-        run_variant(p64, g, wait=True)
-        run_variant(p_mixed, g, wait=True)
-        a = p64.getProbabilities().copy()
-        p64.resultsResideOnHost = False
-        b = p_mixed.getProbabilities().astype(np.float64)
+    for g, p_mixed in zip(grids, props):
+        p64 = make_prop(g, "fp64")
+        p64.calculateProbabilities(g.antinu)
+        p_mixed.calculateProbabilities(g.antinu)
+        d = np.abs(p64.getProbabilities() - p_mixed.getProbabilities().astype(np.float64))
         p_mixed.resultsResideOnHost = False
-        d = np.abs(a - b)
+        p64.close()
         worst = max(worst, float(d.max()))
         sq += float((d * d).sum())
         n += d.size
-    p64.close()
     return worst, float(np.sqrt(sq / n))
 
 
@@ -184,14 +178,59 @@ def run_scan(args, sync):
     return result
 
 
+def run_mcmc(args, sync):
+    """Chain of sequential proposals: each step re-propagates the 8192 x 8192 grid on the
+    100-shell Earth (mixed precision) and reduces (P - D)^2 on the device."""
+    from cudaprob3_amd import metrics as M
+    from cudaprob3_amd import workloads as W
+    from cudaprob3_amd.mcmc import Chain
+
+    grid = W.fine100_grid(args.n if args.n != 2048 else 8192)
+    chain = Chain(grid, W.NUFIT, sigma_p=0.1, seed=5)
+    log(f"[bench] mcmc: {chain.n_pairs} pairs per step, 100 shells")
+    chain.run(args.warmup)
+    chain.stats.step_ms.clear()
+    sync()
+    t0 = time.perf_counter()
+    st = chain.run(args.steps)
+    sync()
+    dt = time.perf_counter() - t0
+    value = args.steps * chain.n_pairs / dt
+    p50, p99 = st.percentile(50), st.percentile(99)
+    chain.prop.calculateProbabilities(grid.antinu)
+    total_ms, k_ms = chain.prop.last_compute_times()
+    ml = M.maxlayers_of(grid.cosines, grid.radii, grid.rhos)
+    flops = M.grid_flops(ml, len(grid.energies))
+    achieved = flops / (k_ms * 1e-3) / 1e12
+    log(f"[bench] {args.steps} steps in {dt:.3f} s -> {value:.4e} pairs/s; step p50 {p50:.3f} ms p99 {p99:.3f} ms; "
+        f"pair kernel {k_ms:.3f} ms; accepted {st.accepted}/{st.steps}")
+    result = {
+        "metric": "pairs/s: Markov-chain steps re-evaluating the full grid (mixed FP32/FP64)",
+        "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (f64 eigenvalues and phase reduction)", "data": "synthetic",
+        "config": {"workload": f"mcmc8192: {len(grid.cosines)} cosZ in [-1,1] x {len(grid.energies)} E log 0.1-1000 GeV, "
+                               "100 equal PREM shells, Gaussian proposals (1% of range), device-side likelihood",
+                   "pairs_per_step": chain.n_pairs},
+        "latency_ms": {"p50": p50, "p99": p99, "mean": float(np.mean(st.step_ms))},
+        "acceptance": st.accepted / max(st.steps, 1),
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": M.PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / M.PEAK_FP32_VALU_TFLOPS, "traffic": None, "kernel": "pairs_mixed_kernel",
+                     "kernel_ms": k_ms, "all_kernels_ms": total_ms, "algorithmic_flops_per_launch": flops},
+    }
+    chain.close()
+    return result
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="jitter12", choices=["jitter12", "scan1024"],
+    ap.add_argument("--workload", default="jitter12", choices=["jitter12", "scan1024", "mcmc8192"],
                     help="jitter12: the 2048^2 grid, four variants per step; scan1024: 1024 parameter sets "
-                         "with uneven grids and Earth models in one persistent launch per step")
+                         "with uneven grids and Earth models in one persistent launch per step; mcmc8192: "
+                         "Markov-chain steps over an 8192^2 grid on a 100-shell Earth")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64", "fp64_plain"])
     ap.add_argument("--n", type=int, default=2048, help="grid edge (cosines = energies = n)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -228,18 +267,24 @@ def main():
     if args.workload == "scan1024":
         print(json.dumps(run_scan(args, sync)), flush=True)
         return
+    if args.workload == "mcmc8192":
+        print(json.dumps(run_mcmc(args, sync)), flush=True)
+        return
 
     grids = W.jitter12_variants(args.n)
     workload = f"jitter12_{args.n}"
     pairs_per_step = sum(g.n_pairs for g in grids)
     log(f"[bench] workload {workload}: {len(grids)} variants x {grids[0].n_pairs} pairs, precision {args.precision}")
 
-    p = make_props(grids, args.precision)
-    dt = time_steps(p, grids, args.steps, args.warmup, sync)
+    props = [make_prop(g, args.precision) for g in grids]
+    shared = torch.cuda.Stream()            # default priority; one stream keeps launches serial
+    for p in props:
+        p.set_stream(shared.cuda_stream)
+    dt = time_steps(props, grids, args.steps, args.warmup, sync)
     value = args.steps * pairs_per_step / dt
     log(f"[bench] {args.steps} steps in {dt:.4f} s -> {value:.4e} pairs/s")
 
-    k_ms, total_ms = kernel_ms(p, grids, reps=3)
+    k_ms, total_ms = kernel_ms(props, grids, reps=3)
     ml = M.maxlayers_of(grids[0].cosines, grids[0].radii, grids[0].rhos)
     flops = M.grid_flops(ml, len(grids[0].energies))
     achieved = flops / (k_ms * 1e-3) / 1e12
@@ -273,7 +318,7 @@ def main():
     }
 
     if not args.no_accuracy and args.precision == "mixed":
-        worst, rms = accuracy(grids, p)
+        worst, rms = accuracy(grids, props)
         result["accuracy_vs_fp64"] = {"max_abs_dP": worst, "rms_dP": rms}
         log(f"[bench] mixed vs fp64: max|dP| {worst:.3e} rms {rms:.3e}")
 
@@ -282,12 +327,13 @@ def main():
         for prec in ("fp64", "fp64_plain"):
             if prec == args.precision:
                 continue
-            q = make_props(grids, prec)
+            qs = [make_prop(g, prec) for g in grids]
             steps = 2
-            dtq = time_steps(q, grids, steps, 1, sync)
+            dtq = time_steps(qs, grids, steps, 1, sync)
             extra[prec] = steps * pairs_per_step / dtq
             log(f"[bench] {prec}: {extra[prec]:.4e} pairs/s")
-            q.close()
+            for q in qs:
+                q.close()
         result["other_precisions_pairs_per_s"] = extra
 
     if not args.no_cpu_baseline:
@@ -295,7 +341,9 @@ def main():
         log(f"[bench] CPU baseline with {threads} threads ...")
         result["cpu_baseline"] = cpu_baseline(grids, threads)
         log(f"[bench] CPU baseline {result['cpu_baseline']['value']:.4e} pairs/s")
-    p.close()
+    for p in props:
+        p.close()
+    del shared
     print(json.dumps(result), flush=True)
 
 

@@ -45,6 +45,9 @@ SIGNATURES = {
     "cp3h_copy_results_f32": (ctypes.c_int, [_handle, _c_float_p, ctypes.c_size_t]),
     "cp3h_device_results": (ctypes.c_int, [_handle, ctypes.POINTER(ctypes.c_void_p), _c_int_p]),
     "cp3h_stream": (ctypes.c_int, [_handle, ctypes.POINTER(ctypes.c_void_p)]),
+    "cp3h_set_stream": (ctypes.c_int, [_handle, ctypes.c_void_p]),
+    "cp3h_copy_results_device": (ctypes.c_int, [_handle, ctypes.c_void_p, ctypes.c_size_t]),
+    "cp3h_sqdiff_results_f32": (ctypes.c_int, [_handle, ctypes.c_void_p, _c_double_p]),
     "cp3h_last_compute_ms": (ctypes.c_int, [_handle, _c_float_p, _c_float_p]),
     "cp3h_last_error": (ctypes.c_char_p, []),
     "cp3h_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p,

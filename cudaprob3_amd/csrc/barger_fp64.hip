@@ -300,6 +300,8 @@ __global__ __launch_bounds__(kBlock) void layer_geometry(GridDesc g, double* __r
 // Q_k = U P_k U^dagger (P_k the eq. 11 products), Hermitian, packed as 9 reals per k:
 // (Q00, Q11, Q22, Re Q01, Im Q01, Re Q02, Im Q02, Re Q12, Im Q12).  Also returns the
 // eigenvalue differences mm and mv of matter_masses.
+// Only Q_k for k >= kFirst are formed (the mixed tables drop Q_0).
+template <int kFirst = 0>
 __device__ inline void flavour_projectors(const OscParams& p, double E, double rho, double Q[3][9],
                                           double mm[3][3], double mv[3][3]) {
     matter_masses(p, E, rho, mm, mv);
@@ -315,7 +317,7 @@ __device__ inline void flavour_projectors(const OscParams& p, double E, double r
             P[0][i][j] = v[0]; P[1][i][j] = v[1]; P[2][i][j] = v[2];
         }
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
+    for (int k = kFirst; k < 3; k++) {
         const int nm[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
 #pragma unroll
         for (int w = 0; w < 6; w++) {
@@ -370,13 +372,13 @@ __global__ __launch_bounds__(kBlock) void shell_spectrum_f64(GridDesc g, OscPara
 // I + (e^{i dphi_1} - 1) Q_1 + (e^{i dphi_2} - 1) Q_2, since sum_k Q_k = I, with
 // dphi_k = -2.534 (lambda_k - lambda_0) L / E.  Stored: Q_1 and Q_2 in float as five float4
 // groups Qm[((s*5 + g)*ne_pad + e)*4 + c] (packed index g*4 + c: 0-8 Q_1, 9-17 Q_2, 18-19
-// zero), and dphi_k / L in turns as Rm[(s*ne_pad + e)*2 + k - 1], double.
+// zero), and dphi_k / L in quarter turns per km as Rm[(s*ne_pad + e)*2 + k - 1], double.
 __device__ inline void spectrum_cell_mixed(const GridDesc& g, const OscParams& p, int ne_pad, int s, int e,
                                            float* __restrict__ Qm, double* __restrict__ Rm) {
     double E, rho;
     slot_inputs(g, s, e, E, rho);
     double q[3][9], mm[3][3], mv[3][3];
-    flavour_projectors(p, E, rho, q, mm, mv);
+    flavour_projectors<1>(p, E, rho, q, mm, mv);
     float packed[20];
 #pragma unroll
     for (int f = 0; f < 9; f++) {
@@ -391,16 +393,24 @@ __device__ inline void spectrum_cell_mixed(const GridDesc& g, const OscParams& p
     for (int grp = 0; grp < 5; grp++)
         dst[(std::size_t(s) * 5 + grp) * st + e] =
             make_float4(packed[4 * grp], packed[4 * grp + 1], packed[4 * grp + 2], packed[4 * grp + 3]);
-    const double turn = 1.0 / (E * (2.0 * kPi));
-    Rm[(std::size_t(s) * st + e) * 2 + 0] = -kLoE * mm[1][0] * turn;
-    Rm[(std::size_t(s) * st + e) * 2 + 1] = -kLoE * mm[2][0] * turn;
+    const double quarter_turn = 2.0 / (E * kPi);      // radians -> quarter turns
+    Rm[(std::size_t(s) * st + e) * 2 + 0] = -kLoE * mm[1][0] * quarter_turn;
+    Rm[(std::size_t(s) * st + e) * 2 + 1] = -kLoE * mm[2][0] * quarter_turn;
 }
 
-__global__ __launch_bounds__(kBlock) void shell_spectrum_mixed(GridDesc g, OscParams p, int ne_pad,
-                                                               float* __restrict__ Qm, double* __restrict__ Rm) {
+// Set-up of one mixed-precision grid in one launch: cells [0, G) are layer lengths
+// (cosine, layer), [G, G + S) spectrum cells (slot, energy).
+__global__ __launch_bounds__(kBlock) void grid_setup_mixed(GridDesc g, OscParams p, int ne_pad, double* __restrict__ geom,
+                                                           float* __restrict__ Qm, double* __restrict__ Rm) {
+    const int slots = g.rows + 1;
+    const int G = g.n_c * slots, S = slots * ne_pad;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (g.rows + 1) * ne_pad) return;
-    spectrum_cell_mixed(g, p, ne_pad, t / ne_pad, t % ne_pad, Qm, Rm);
+    if (t < G) {
+        geom[t] = geometry_cell(g, t / slots, t % slots);
+    } else if (t < G + S) {
+        const int u = t - G;
+        spectrum_cell_mixed(g, p, ne_pad, u / ne_pad, u % ne_pad, Qm, Rm);
+    }
 }
 
 // Set index of work cell t given the per-set prefix sums off[0..n] (off[0] = 0).
@@ -456,11 +466,11 @@ hipError_t launch_geometry(const GridDesc& g, double* geom, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_spectrum_mixed(const GridDesc& g, const OscParams& p, int ne_pad, float* Qm, double* Rm,
-                                 hipStream_t s) {
-    const unsigned n = unsigned(g.rows + 1) * ne_pad;
-    if (n == 0 || g.n_e == 0) return hipSuccess;
-    hipLaunchKernelGGL(shell_spectrum_mixed, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g, p, ne_pad,
+hipError_t launch_setup_mixed(const GridDesc& g, const OscParams& p, int ne_pad, double* geom, float* Qm, double* Rm,
+                              hipStream_t s) {
+    if (g.n_c == 0 || g.n_e == 0) return hipSuccess;
+    const unsigned n = unsigned(g.n_c) * (g.rows + 1) + unsigned(g.rows + 1) * ne_pad;
+    hipLaunchKernelGGL(grid_setup_mixed, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g, p, ne_pad, geom,
                        Qm, Rm);
     return hipGetLastError();
 }

@@ -65,10 +65,12 @@ hipError_t launch_geometry(const GridDesc& g, double* geom, hipStream_t s);
 //  FP64:  Q[(s*27 + k*9 + f)*ne_pad + e] (Q_k packed, 9 reals), R[(s*3 + k)*ne_pad + e] =
 //         -2.534 lambda_k (phase R*L/E).
 //  mixed: Qm float4 groups [(s*5 + g)*ne_pad + e] holding Q_1, Q_2 (18 floats + 2 zero),
-//         Rm[(s*ne_pad + e)*2 + k-1] = -2.534 (lambda_k - lambda_0) / (2 pi E) (turns per km).
+//         Rm[(s*ne_pad + e)*2 + k-1] = -2.534 (lambda_k - lambda_0) / E in quarter turns per km.
 constexpr int kMixedQFloats = 20;
 constexpr int kMixedRDoubles = 2;
-hipError_t launch_spectrum_mixed(const GridDesc& g, const OscParams& p, int ne_pad, float* Qm, double* Rm, hipStream_t s);
+// Mixed set-up in one launch: layer lengths (geom) and the mixed spectrum table.
+hipError_t launch_setup_mixed(const GridDesc& g, const OscParams& p, int ne_pad, double* geom, float* Qm, double* Rm,
+                              hipStream_t s);
 hipError_t launch_spectrum_f64(const GridDesc& g, const OscParams& p, int ne_pad, double* Q, double* R, hipStream_t s);
 
 // Table-driven per-pair kernels.
@@ -76,6 +78,13 @@ hipError_t launch_pairs_mixed(const GridDesc& g, int ne_pad, const double* geom,
                               const double* R, float* out, hipStream_t s);
 hipError_t launch_pairs_f64(const GridDesc& g, int ne_pad, const double* geom, const double* Q,
                             const double* R, double* out, hipStream_t s);
+
+// ---- device reductions --------------------------------------------------------------------
+
+// Number of per-workgroup partial sums launch_sqdiff_sum needs as scratch (plus one slot).
+int reduce_partials_needed();
+// sum_i (a_i - b_i)^2 in FP64 for device floats; result in partials[reduce_partials_needed()].
+hipError_t launch_sqdiff_sum(const float* a, const float* b, std::size_t n, double* partials, hipStream_t s);
 
 // ---- batched parameter scans ------------------------------------------------------------
 

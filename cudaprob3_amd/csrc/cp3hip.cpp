@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -22,8 +23,11 @@ using cp3host::vacuum_order;
 
 struct cp3h_propagator {
     int device = 0, n_c = 0, n_e = 0, precision = CP3H_FP64_PLAIN, ne_pad = 0, rows = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // the stream work is enqueued on
+    hipStream_t own_stream = nullptr;  // created and destroyed by the handle
     hipEvent_t ev_start = nullptr, ev_pairs = nullptr, ev_stop = nullptr;
+    double* d_partials = nullptr;      // reduction scratch (cp3h_sqdiff_results_f32)
+    double* h_scalar = nullptr;        // pinned landing slot for a device scalar
     double* d_cos = nullptr;
     double* d_energies = nullptr;
     int* d_maxlayers = nullptr;
@@ -105,7 +109,8 @@ int cp3h_create(int device, int n_cosines, int n_energies, int precision, cp3h_p
 #define CP3H_TRY(call) do { int rc_ = (call); if (rc_ != CP3H_OK) return bail(rc_); } while (0)
     CP3H_TRY([&]() -> int {
         CP3H_HIP(hipSetDevice(device));
-        CP3H_HIP(hipStreamCreate(&h->stream));
+        CP3H_HIP(hipStreamCreate(&h->own_stream));
+        h->stream = h->own_stream;
         CP3H_HIP(hipEventCreate(&h->ev_start));
         CP3H_HIP(hipEventCreate(&h->ev_pairs));
         CP3H_HIP(hipEventCreate(&h->ev_stop));
@@ -133,10 +138,12 @@ int cp3h_destroy(cp3h_propagator* h) {
     free_dev(h->d_Q);
     free_dev(h->d_R);
     free_dev(h->d_out);
+    free_dev(h->d_partials);
+    if (h->h_scalar) (void)hipHostFree(h->h_scalar);
     if (h->ev_start) (void)hipEventDestroy(h->ev_start);
     if (h->ev_pairs) (void)hipEventDestroy(h->ev_pairs);
     if (h->ev_stop) (void)hipEventDestroy(h->ev_stop);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     delete h;
     return CP3H_OK;
 }
@@ -278,13 +285,14 @@ int cp3h_compute_async(cp3h_propagator* h, int antineutrino, double production_h
         CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
         CP3H_HIP(cp3::launch_plain_f64(g, h->params, static_cast<double*>(h->d_out), h->stream));
     } else {
-        CP3H_HIP(cp3::launch_geometry(g, h->d_geom, h->stream));
         if (h->precision == CP3H_MIXED) {
-            CP3H_HIP(cp3::launch_spectrum_mixed(g, h->params, h->ne_pad, static_cast<float*>(h->d_Q), h->d_R, h->stream));
+            CP3H_HIP(cp3::launch_setup_mixed(g, h->params, h->ne_pad, h->d_geom, static_cast<float*>(h->d_Q), h->d_R,
+                                             h->stream));
             CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
             CP3H_HIP(cp3::launch_pairs_mixed(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_R,
                                              static_cast<float*>(h->d_out), h->stream));
         } else {
+            CP3H_HIP(cp3::launch_geometry(g, h->d_geom, h->stream));
             CP3H_HIP(cp3::launch_spectrum_f64(g, h->params, h->ne_pad, static_cast<double*>(h->d_Q), h->d_R, h->stream));
             CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
             CP3H_HIP(cp3::launch_pairs_f64(g, h->ne_pad, h->d_geom, static_cast<const double*>(h->d_Q), h->d_R,
@@ -350,6 +358,43 @@ int cp3h_stream(cp3h_propagator* h, void** stream) {
     if (int rc = check_handle(h)) return rc;
     if (!stream) return fail(CP3H_ERR_ARGUMENT, "null output pointer");
     *stream = static_cast<void*>(h->stream);
+    return CP3H_OK;
+}
+
+int cp3h_set_stream(cp3h_propagator* h, void* stream) {
+    if (int rc = check_handle(h)) return rc;
+    if (int rc = sync_and_select(h)) return rc;
+    h->stream = stream ? static_cast<hipStream_t>(stream) : h->own_stream;
+    return CP3H_OK;
+}
+
+int cp3h_copy_results_device(cp3h_propagator* h, void* dst_device, size_t count) {
+    if (int rc = check_handle(h)) return rc;
+    if (count != h->n_results()) return fail(CP3H_ERR_ARGUMENT, "result count must be 9*n_cosines*n_energies");
+    if (count > 0 && !dst_device) return fail(CP3H_ERR_ARGUMENT, "null destination");
+    CP3H_HIP(hipSetDevice(h->device));
+    if (count) CP3H_HIP(hipMemcpyAsync(dst_device, h->d_out, count * h->out_bytes, hipMemcpyDeviceToDevice, h->stream));
+    CP3H_HIP(hipStreamSynchronize(h->stream));
+    return CP3H_OK;
+}
+
+int cp3h_sqdiff_results_f32(cp3h_propagator* h, const float* data_device, double* result) {
+    if (int rc = check_handle(h)) return rc;
+    if (h->out_bytes != 4) return fail(CP3H_ERR_STATE, "sqdiff_results_f32 needs float results (CP3H_MIXED)");
+    if (!data_device || !result) return fail(CP3H_ERR_ARGUMENT, "null pointer");
+    if (reinterpret_cast<std::uintptr_t>(data_device) % 16 != 0)
+        return fail(CP3H_ERR_ARGUMENT, "data must be 16-byte aligned");
+    CP3H_HIP(hipSetDevice(h->device));
+    if (!h->d_partials) {
+        CP3H_HIP(hipMalloc(&h->d_partials, sizeof(double) * (cp3::reduce_partials_needed() + 1)));
+        CP3H_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->h_scalar), sizeof(double)));
+    }
+    CP3H_HIP(cp3::launch_sqdiff_sum(static_cast<const float*>(h->d_out), data_device, h->n_results(), h->d_partials,
+                                    h->stream));
+    CP3H_HIP(hipMemcpyAsync(h->h_scalar, h->d_partials + cp3::reduce_partials_needed(), sizeof(double),
+                            hipMemcpyDeviceToHost, h->stream));
+    CP3H_HIP(hipStreamSynchronize(h->stream));
+    *result = *h->h_scalar;
     return CP3H_OK;
 }
 

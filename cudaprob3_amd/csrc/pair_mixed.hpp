@@ -2,7 +2,8 @@
 // batched kernels.  Layer algebra and table layout: see propagate_pairs.hip.
 //   L   : shells crossed (maxlayers of the row)
 //   gl  : layer lengths of the row in km, gl[0..L]
-//   Q, R: the energy tile's spectrum table (float4 groups / double2 phase rates), row stride st
+//   Q, R: the energy tile's spectrum table (float4 groups / double2 phase rates in quarter turns
+//         per km), row stride st
 //   ie  : energy index (< ne_pad)
 // Writes p[in][out] = P(in -> out).
 #pragma once
@@ -13,6 +14,41 @@ namespace cp3 {
 template <typename T> struct Cx { T re, im; };
 
 __device__ inline float clamp01(float x) { return fminf(1.0f, fmaxf(0.0f, x)); }
+
+// acc += a * b for complex a, b as four fused multiply-adds (written out so the compiler
+// cannot split them into separate multiplies and adds).
+__device__ inline void cmac(Cx<float>& acc, Cx<float> a, Cx<float> b) {
+    acc.re = __builtin_fmaf(a.re, b.re, acc.re);
+    acc.re = __builtin_fmaf(-a.im, b.im, acc.re);
+    acc.im = __builtin_fmaf(a.re, b.im, acc.im);
+    acc.im = __builtin_fmaf(a.im, b.re, acc.im);
+}
+
+__device__ inline Cx<float> cmul(Cx<float> a, Cx<float> b) {
+    Cx<float> r{a.re * b.re, a.re * b.im};
+    r.re = __builtin_fmaf(-a.im, b.im, r.re);
+    r.im = __builtin_fmaf(a.im, b.re, r.im);
+    return r;
+}
+
+// e^{i phi} - 1 for a phase given in quarter turns (phi = t * pi/2), as (cos phi - 1, sin phi).
+// Cody-Waite style reduction in FP64: q = rint(t), r = t - q in [-1/2, 1/2] is exact, so the
+// phase reaches FP32 only as the small remainder z = r * pi/2 in [-pi/4, pi/4]; there
+// sin z and cos z - 1 are Taylor polynomials (next terms below 3e-9 relative) evaluated in
+// FP32 by Horner's rule, and the quadrant q mod 4 rotates the result.
+__device__ inline void phase_factor_m1(double t, float& cm1, float& sn) {
+    const double q = rint(t);
+    const float z = float((t - q) * 1.5707963267948966);
+    const float z2 = z * z;
+    const float s0 = z + z * z2 * (-1.66666667e-1f + z2 * (8.33333333e-3f + z2 * (-1.98412698e-4f + z2 * 2.75573192e-6f)));
+    const float c0m1 = z2 * (-0.5f + z2 * (4.16666667e-2f + z2 * (-1.38888889e-3f + z2 * (2.48015873e-5f + z2 * -2.75573192e-7f))));
+    switch (int(q) & 3) {
+        case 0: cm1 = c0m1; sn = s0; break;                  // cos z, sin z
+        case 1: cm1 = -s0 - 1.0f; sn = c0m1 + 1.0f; break;   // -sin z, cos z
+        case 2: cm1 = -c0m1 - 2.0f; sn = -s0; break;         // -cos z, -sin z
+        default: cm1 = s0 - 1.0f; sn = -(c0m1 + 1.0f); break; // sin z, -cos z
+    }
+}
 
 __device__ inline void propagate_pair_mixed(int L, const double* __restrict__ gl, const float4* __restrict__ Q,
                                             const double2* __restrict__ R, std::size_t st, int ie,
@@ -30,33 +66,33 @@ __device__ inline void propagate_pair_mixed(int L, const double* __restrict__ gl
         }
         const double2 r = R[std::size_t(i) * st + ie];
         float wr[2], wi[2];
-        {
-            double t1 = r.x * dist, t2 = r.y * dist;
-            t1 -= rint(t1);
-            t2 -= rint(t2);
-            const float x1 = float(t1), x2 = float(t2);
-            wr[0] = __builtin_amdgcn_cosf(x1) - 1.0f;
-            wi[0] = __builtin_amdgcn_sinf(x1);
-            wr[1] = __builtin_amdgcn_cosf(x2) - 1.0f;
-            wi[1] = __builtin_amdgcn_sinf(x2);
-        }
-        // A = I + w_1 Q_1 + w_2 Q_2
+        phase_factor_m1(r.x * dist, wr[0], wi[0]);
+        phase_factor_m1(r.y * dist, wr[1], wi[1]);
+        // A = I + w_1 Q_1 + w_2 Q_2; Q Hermitian: A_ji uses conj(Q_ij)
         Cx<float> A[3][3];
 #pragma unroll
-        for (int d = 0; d < 3; d++)
-            A[d][d] = Cx<float>{1.0f + wr[0] * q[d] + wr[1] * q[9 + d], wi[0] * q[d] + wi[1] * q[9 + d]};
+        for (int d = 0; d < 3; d++) {
+            A[d][d].re = __builtin_fmaf(wr[1], q[9 + d], __builtin_fmaf(wr[0], q[d], 1.0f));
+            A[d][d].im = __builtin_fmaf(wi[1], q[9 + d], wi[0] * q[d]);
+        }
         const int idx[3][2] = {{0, 1}, {0, 2}, {1, 2}};
 #pragma unroll
         for (int w = 0; w < 3; w++) {
-            float ure = 0.0f, uim = 0.0f, lre = 0.0f, lim = 0.0f;
-#pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const float re = q[9 * k + 3 + 2 * w], im = q[9 * k + 4 + 2 * w];
-                ure += wr[k] * re - wi[k] * im;
-                uim += wr[k] * im + wi[k] * re;
-                lre += wr[k] * re + wi[k] * im;
-                lim += wi[k] * re - wr[k] * im;
-            }
+            const float a0 = q[3 + 2 * w], b0 = q[4 + 2 * w], a1 = q[12 + 2 * w], b1 = q[13 + 2 * w];
+            // w_k (a + i b) and w_k (a - i b), summed over k
+            float ure = wr[0] * a0, uim = wr[0] * b0, lre = ure, lim = wi[0] * a0;
+            ure = __builtin_fmaf(-wi[0], b0, ure);
+            uim = __builtin_fmaf(wi[0], a0, uim);
+            lre = __builtin_fmaf(wi[0], b0, lre);
+            lim = __builtin_fmaf(-wr[0], b0, lim);
+            ure = __builtin_fmaf(wr[1], a1, ure);
+            ure = __builtin_fmaf(-wi[1], b1, ure);
+            uim = __builtin_fmaf(wr[1], b1, uim);
+            uim = __builtin_fmaf(wi[1], a1, uim);
+            lre = __builtin_fmaf(wr[1], a1, lre);
+            lre = __builtin_fmaf(wi[1], b1, lre);
+            lim = __builtin_fmaf(wi[1], a1, lim);
+            lim = __builtin_fmaf(-wr[1], b1, lim);
             A[idx[w][0]][idx[w][1]] = Cx<float>{ure, uim};
             A[idx[w][1]][idx[w][0]] = Cx<float>{lre, lim};
         }
@@ -71,13 +107,10 @@ __device__ inline void propagate_pair_mixed(int L, const double* __restrict__ gl
             for (int x = 0; x < 3; x++)
 #pragma unroll
                 for (int y = 0; y < 2; y++) {
-                    float re = 0.0f, im = 0.0f;
-#pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        re += A[x][k].re * F[k][y].re - A[x][k].im * F[k][y].im;
-                        im += A[x][k].re * F[k][y].im + A[x][k].im * F[k][y].re;
-                    }
-                    G[x][y] = Cx<float>{re, im};
+                    Cx<float> acc = cmul(A[x][0], F[0][y]);
+                    cmac(acc, A[x][1], F[1][y]);
+                    cmac(acc, A[x][2], F[2][y]);
+                    G[x][y] = acc;
                 }
 #pragma unroll
             for (int x = 0; x < 3; x++)
@@ -95,13 +128,10 @@ __device__ inline void propagate_pair_mixed(int L, const double* __restrict__ gl
                     for (int x = 0; x < 2; x++)
 #pragma unroll
                         for (int y = 0; y < 3; y++) {
-                            float re = 0.0f, im = 0.0f;
-#pragma unroll
-                            for (int k = 0; k < 3; k++) {
-                                re += M[x][k].re * A[k][y].re - M[x][k].im * A[k][y].im;
-                                im += M[x][k].re * A[k][y].im + M[x][k].im * A[k][y].re;
-                            }
-                            N[x][y] = Cx<float>{re, im};
+                            Cx<float> acc = cmul(M[x][0], A[0][y]);
+                            cmac(acc, M[x][1], A[1][y]);
+                            cmac(acc, M[x][2], A[2][y]);
+                            N[x][y] = acc;
                         }
 #pragma unroll
                     for (int x = 0; x < 2; x++)
@@ -118,13 +148,10 @@ __device__ inline void propagate_pair_mixed(int L, const double* __restrict__ gl
         for (int x = 0; x < 2; x++)
 #pragma unroll
             for (int y = 0; y < 2; y++) {
-                float re = 0.0f, im = 0.0f;
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    re += M[x][k].re * F[k][y].re - M[x][k].im * F[k][y].im;
-                    im += M[x][k].re * F[k][y].im + M[x][k].im * F[k][y].re;
-                }
-                S[x][y] = Cx<float>{re, im};
+                Cx<float> acc = cmul(M[x][0], F[0][y]);
+                cmac(acc, M[x][1], F[1][y]);
+                cmac(acc, M[x][2], F[2][y]);
+                S[x][y] = acc;
             }
     } else {
 #pragma unroll
@@ -136,7 +163,7 @@ __device__ inline void propagate_pair_mixed(int L, const double* __restrict__ gl
 #pragma unroll
     for (int in = 0; in < 2; in++)
 #pragma unroll
-        for (int o = 0; o < 2; o++) p[in][o] = S[o][in].re * S[o][in].re + S[o][in].im * S[o][in].im;
+        for (int o = 0; o < 2; o++) p[in][o] = __builtin_fmaf(S[o][in].re, S[o][in].re, S[o][in].im * S[o][in].im);
     p[0][2] = clamp01(1.0f - p[0][0] - p[0][1]);
     p[1][2] = clamp01(1.0f - p[1][0] - p[1][1]);
     p[2][0] = clamp01(1.0f - p[0][0] - p[1][0]);

@@ -15,9 +15,9 @@
 //  * The layer's global phase e^{i phi_0} is dropped (it cancels in |amplitude|^2), and
 //    sum_k Q_k = I gives A' = I + (e^{i dphi_1} - 1) Q_1 + (e^{i dphi_2} - 1) Q_2 with
 //    dphi_k = phi_k - phi_0.  dphi_k (up to ~10^3 rad on core-crossing paths at 0.05 GeV,
-//    where FP32 would lose 1e-4 rad) is formed and reduced modulo one turn in FP64; the
-//    hardware sin/cos (v_sin_f32 / v_cos_f32, argument in turns) and the matrix arithmetic
-//    are FP32.
+//    where FP32 would lose 1e-4 rad) is formed and reduced to [-pi/4, pi/4] plus a quadrant
+//    in FP64; sin/cos of the remainder (polynomials, pair_mixed.hpp) and the matrix
+//    arithmetic are FP32.
 //  * The total amplitude is unitary, so its upper-left 2x2 block fixes all nine
 //    probabilities (rows and columns of |S|^2 sum to one).  Only columns 0-1 of the forward
 //    product and rows 0-1 of the mantle product are carried: 18 + 18 complex MACs per

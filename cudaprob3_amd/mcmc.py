@@ -1,0 +1,101 @@
+"""Markov-chain parameter fit over a fixed (cosZ, E) grid on one MI355X.
+
+Each step proposes new oscillation parameters (Gaussian, sigma = 1 % of each parameter's
+range), re-propagates the whole grid with the mixed-precision kernel, compares it with
+pseudo-data on the device (sum of squared differences in FP64, cp3h_sqdiff_results_f32) and
+accepts or rejects with the Metropolis rule.  Only one double per step crosses PCIe.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from . import workloads as W
+from .propagator import HipPropagatorSingle
+
+PARAM_RANGE = W.PARAM_HI - W.PARAM_LO
+
+
+@dataclass
+class ChainStats:
+    steps: int = 0
+    accepted: int = 0
+    step_ms: list = field(default_factory=list)
+    loglike: list = field(default_factory=list)
+
+    def percentile(self, q):
+        return float(np.percentile(self.step_ms, q)) if self.step_ms else float("nan")
+
+
+class Chain:
+    def __init__(self, grid: W.Grid, true_params, sigma_p: float = 0.01, seed: int = 5, device: int = 0):
+        import torch   # device memory for the pseudo-data only
+
+        self.grid = grid
+        self.sigma_p = sigma_p
+        self.rng = np.random.Generator(np.random.PCG64(seed))
+        self.prop = HipPropagatorSingle(device, len(grid.cosines), len(grid.energies), precision="mixed")
+        p = self.prop
+        p.setEnergyList(grid.energies)
+        p.setCosineList(grid.cosines)
+        p.setDensity(grid.radii, grid.rhos)
+        p.setProductionHeight(grid.h_km)
+        self._lib = _lib.load()
+        n = 9 * len(grid.cosines) * len(grid.energies)
+        self.data = torch.empty(n, dtype=torch.float32, device=f"cuda:{device}")
+        self._evaluate(true_params)
+        _lib.check(self._lib.cp3h_copy_results_device(p._handle(), ctypes.c_void_p(self.data.data_ptr()), n))
+        self.params = np.array(true_params, dtype=np.float64)
+        self.current = self.loglike(self.params)
+        self.stats = ChainStats()
+
+    @property
+    def n_pairs(self):
+        return len(self.grid.cosines) * len(self.grid.energies)
+
+    def _evaluate(self, params):
+        p = self.prop
+        p.setMNSMatrix(*params[:4])
+        p.setNeutrinoMasses(*params[4:])
+        p.calculateProbabilitiesAsync(self.grid.antinu)
+
+    def loglike(self, params) -> float:
+        """-1/2 sum (P - D)^2 / sigma_P^2 over all nine channels and every pair."""
+        self._evaluate(params)
+        r = ctypes.c_double()
+        _lib.check(self._lib.cp3h_sqdiff_results_f32(self.prop._handle(), ctypes.c_void_p(self.data.data_ptr()),
+                                                     ctypes.byref(r)))
+        return -0.5 * r.value / self.sigma_p ** 2
+
+    def propose(self):
+        q = self.params + self.rng.normal(0.0, 0.01 * PARAM_RANGE)
+        q[3] = math.fmod(q[3], 2 * math.pi)
+        if q[3] < 0:
+            q[3] += 2 * math.pi
+        return q
+
+    def step(self):
+        t0 = time.perf_counter()
+        q = self.propose()
+        ll = self.loglike(q)
+        accept = math.log(self.rng.random() + 1e-300) < ll - self.current
+        if accept:
+            self.params, self.current = q, ll
+            self.stats.accepted += 1
+        self.stats.steps += 1
+        self.stats.step_ms.append((time.perf_counter() - t0) * 1e3)
+        self.stats.loglike.append(self.current)
+        return accept
+
+    def run(self, steps: int):
+        for _ in range(steps):
+            self.step()
+        return self.stats
+
+    def close(self):
+        self.prop.close()

@@ -272,10 +272,25 @@ class HipPropagatorSingle(Propagator):
             _lib.check(self._lib.cp3h_set_maxlayers(self._h, _ptr(m, ctypes.c_int), len(m)))
 
     # ---- compute (cudapropagator.cuh:162-226) -------------------------------------------
+    def setMNSMatrix(self, theta12, theta13, theta23, dCP):
+        super().setMNSMatrix(theta12, theta13, theta23, dCP)
+        self._params_uploaded = False
+
+    def setNeutrinoMasses(self, dm12sq, dm23sq):
+        super().setNeutrinoMasses(dm12sq, dm23sq)
+        self._params_uploaded = False
+
     def calculateProbabilitiesAsync(self, type_):
         h = self._handle()
         if not self.isSetProductionHeight:
             raise RuntimeError("HipPropagatorSingle::calculateProbabilities. production height was not set")
+        self.resultsResideOnHost = False
+        if not getattr(self, "_params_uploaded", False):
+            self._upload_params(h)
+            self._params_uploaded = True
+        _lib.check(self._lib.cp3h_compute_async(h, int(type_), float(self.ProductionHeightinCentimeter)))
+
+    def _upload_params(self, h):
         if self.dtype == np.float64:
             U = np.ascontiguousarray(self.Mix_U, dtype=np.float64)
             D = np.ascontiguousarray(self.dm, dtype=np.float64)
@@ -284,8 +299,6 @@ class HipPropagatorSingle(Propagator):
             U = np.ascontiguousarray(self.Mix_U, dtype=np.float32)
             D = np.ascontiguousarray(self.dm, dtype=np.float32)
             _lib.check(self._lib.cp3h_set_params_f32(h, _ptr(U, ctypes.c_float), _ptr(D, ctypes.c_float)))
-        self.resultsResideOnHost = False
-        _lib.check(self._lib.cp3h_compute_async(h, int(type_), float(self.ProductionHeightinCentimeter)))
 
     def waitForCompletion(self):
         _lib.check(self._lib.cp3h_wait(self._handle()))
@@ -326,6 +339,11 @@ class HipPropagatorSingle(Propagator):
         p, b = ctypes.c_void_p(), ctypes.c_int()
         _lib.check(self._lib.cp3h_device_results(self._handle(), ctypes.byref(p), ctypes.byref(b)))
         return p.value, b.value
+
+    def set_stream(self, stream_ptr: int | None):
+        """Enqueue on the caller's hipStream_t (an int handle, e.g. torch.cuda.Stream().cuda_stream);
+        None returns to the propagator's own stream."""
+        _lib.check(self._lib.cp3h_set_stream(self._handle(), ctypes.c_void_p(stream_ptr or None)))
 
     def last_compute_ms(self) -> float:
         """GPU milliseconds of all kernels of the last compute."""

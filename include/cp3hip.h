@@ -102,6 +102,18 @@ int cp3h_device_results(cp3h_propagator* h, const void** ptr, int* elem_bytes);
 /* The handle's hipStream_t (as void*), for callers that order their own work after it. */
 int cp3h_stream(cp3h_propagator* h, void** stream);
 
+/* Run the handle's work on the caller's hipStream_t (as void*, created on the handle's
+   device; NULL returns to the handle's own stream).  The caller keeps ownership. */
+int cp3h_set_stream(cp3h_propagator* h, void* stream);
+
+/* Copy the results (SoA, the handle's element type) to device memory of the same size. */
+int cp3h_copy_results_device(cp3h_propagator* h, void* dst_device, size_t count);
+
+/* sum over all 9*n_c*n_e results of (result - data)^2, accumulated in FP64 on the device;
+   data is a device float array in the results' SoA layout (CP3H_MIXED handles).  Orders
+   after the last compute on the handle's stream and waits for the scalar. */
+int cp3h_sqdiff_results_f32(cp3h_propagator* h, const float* data_device, double* result);
+
 /* GPU time of the last compute, from HIP events on the handle's stream (waits for it):
    total_ms covers all of its kernels, pairs_ms only the per-pair propagation kernel
    (for CP3H_FP64_PLAIN the two are the same kernel).  Either pointer may be NULL. */

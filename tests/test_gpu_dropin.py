@@ -17,10 +17,10 @@ BIN = os.path.join(ROOT, "tests", "cpp", "_bin", "dropin")
 
 def run_dropin(tmp_path, cos, E, model, params, h_km, antinu, type_, precision):
     dens = tmp_path / "model.dat"
-    dens.write_text("\n".join(f"{r!r} {d!r}" for r, d in model) + "\n")
+    dens.write_text("\n".join(f"{float(r)!r} {float(d)!r}" for r, d in model) + "\n")
     out = tmp_path / "P.bin"
     spec = tmp_path / "spec.txt"
-    lines = [f"{len(cos)} {len(E)} {antinu} {h_km!r} {type_} {precision}",
+    lines = [f"{len(cos)} {len(E)} {antinu} {float(h_km)!r} {type_} {precision}",
              " ".join(repr(float(x)) for x in params),
              " ".join(repr(float(x)) for x in cos),
              " ".join(repr(float(x)) for x in E),
