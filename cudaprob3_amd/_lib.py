@@ -81,12 +81,24 @@ class Cp3hError(RuntimeError):
         self.status = status
 
 
+def _settle_hip_runtime() -> None:
+    """PyTorch wheels bundle their own HIP runtime under the same soname as ROCm's
+    (libamdhip64.so.7).  Whichever is loaded first serves the whole process, and PyTorch
+    cannot initialise on a foreign one, so when PyTorch is installed it is loaded before
+    libcp3hip.so; both then share PyTorch's runtime (one runtime per process)."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load() -> ctypes.CDLL:
     global _LIB
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() first "
                                "(the propagation path has no CPU fallback)")
+        _settle_hip_runtime()
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)
