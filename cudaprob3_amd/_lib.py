@@ -68,6 +68,21 @@ SIGNATURES = {
     "cp3h_batch_last_ms": (ctypes.c_int, [_handle, _c_float_p, _c_float_p]),
     "cp3h_batch_copy_results_f32": (ctypes.c_int, [_handle, ctypes.c_int, _c_float_p, ctypes.c_size_t]),
     "cp3h_batch_device_results": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "cp3h_batch_bin_average_f32": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_float_p,
+                                                  ctypes.c_void_p]),
+    "cp3h_mlp_param_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t)]),
+    "cp3h_mlp_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_handle)]),
+    "cp3h_mlp_destroy": (ctypes.c_int, [_handle]),
+    "cp3h_mlp_set_stream": (ctypes.c_int, [_handle, ctypes.c_void_p]),
+    "cp3h_mlp_set_scaling": (ctypes.c_int, [_handle, _c_float_p, _c_float_p]),
+    "cp3h_mlp_set_params": (ctypes.c_int, [_handle, _c_float_p, ctypes.c_size_t]),
+    "cp3h_mlp_get_params": (ctypes.c_int, [_handle, _c_float_p, ctypes.c_size_t]),
+    "cp3h_mlp_forward": (ctypes.c_int, [_handle, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p]),
+    "cp3h_mlp_gradients": (ctypes.c_int, [_handle, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_float_p,
+                                          ctypes.c_size_t, _c_float_p]),
+    "cp3h_mlp_train_step": (ctypes.c_int, [_handle, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float,
+                                           _c_float_p]),
+    "cp3h_mlp_sync": (ctypes.c_int, [_handle]),
 }
 
 _LIB: ctypes.CDLL | None = None

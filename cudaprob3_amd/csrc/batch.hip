@@ -11,6 +11,8 @@
 #include "common.hpp"
 #include "pair_mixed.hpp"
 
+#include <algorithm>
+
 namespace cp3 {
 namespace {
 
@@ -52,7 +54,39 @@ __global__ __launch_bounds__(kWaves * kWave) void batch_pairs(BatchDesc b) {
     }
 }
 
+struct BinWeights {
+    float w[kMaxBinNodes];
+};
+
+// out[(t * n_c + ic) * n_bins + b] = sum_g w[g] * P[t][ic][b * nodes + g] for one set
+__global__ __launch_bounds__(kBlock) void bin_average(const float* __restrict__ P, int n_c, int n_e, int nodes,
+                                                      BinWeights w, float* __restrict__ out) {
+    const int n_bins = n_e / nodes;
+    const long long total = 9LL * n_c * n_bins;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long row = i / n_bins;                 // t * n_c + ic
+        const int bin = int(i % n_bins);
+        const float* src = P + row * n_e + (long long)bin * nodes;
+        float acc = 0.0f;
+        for (int g = 0; g < nodes; g++) acc = __builtin_fmaf(w.w[g], src[g], acc);
+        out[i] = acc;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_bin_average(const float* P, int n_c, int n_e, int nodes, const float* weights, float* out,
+                              hipStream_t s) {
+    if (nodes < 1 || nodes > kMaxBinNodes || n_e % nodes != 0) return hipErrorInvalidValue;
+    BinWeights w{};
+    for (int g = 0; g < nodes; g++) w.w[g] = weights[g];
+    const long long total = 9LL * n_c * (n_e / nodes);
+    if (total == 0) return hipSuccess;
+    const unsigned blocks = unsigned(std::min<long long>((total + kBlock - 1) / kBlock, 65535));
+    hipLaunchKernelGGL(bin_average, dim3(blocks), dim3(kBlock), 0, s, P, n_c, n_e, nodes, w, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_batch_pairs(const BatchDesc& b, int blocks, int waves, hipStream_t s) {
     if (blocks <= 0 || blocks % kXcds != 0) return hipErrorInvalidValue;

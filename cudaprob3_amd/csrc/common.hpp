@@ -130,4 +130,10 @@ hipError_t launch_batch_setup(const BatchDesc& b, int blocks, hipStream_t s);
 // (waves in {4, 8, 16}); each wavefront takes (set, cosine row, 64-energy tile) items.
 hipError_t launch_batch_pairs(const BatchDesc& b, int blocks, int waves, hipStream_t s);
 
+// Energy-bin average of one set's SoA results whose n_e energies are `nodes` quadrature nodes
+// per bin: out[9][n_c][n_e / nodes].
+constexpr int kMaxBinNodes = 32;
+hipError_t launch_bin_average(const float* P, int n_c, int n_e, int nodes, const float* weights, float* out,
+                              hipStream_t s);
+
 }  // namespace cp3

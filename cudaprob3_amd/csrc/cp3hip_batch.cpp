@@ -403,6 +403,25 @@ int cp3h_batch_copy_results_f32(cp3h_batch* b, int set, float* dst, size_t count
     return CP3H_OK;
 }
 
+int cp3h_batch_bin_average_f32(cp3h_batch* b, int first, int count, int nodes, const float* weights,
+                               float* out_device) {
+    if (!b) return fail(CP3H_ERR_ARGUMENT, "null batch handle");
+    if (first < 0 || count < 0 || first + count > b->n_sets) return fail(CP3H_ERR_ARGUMENT, "set range out of bounds");
+    if (nodes < 1 || nodes > cp3::kMaxBinNodes || !weights || (count > 0 && !out_device))
+        return fail(CP3H_ERR_ARGUMENT, "bad bin-average arguments");
+    CP3H_HIP(hipSetDevice(b->device));
+    std::size_t off = 0;
+    for (int i = first; i < first + count; i++) {
+        const cp3::BatchSet& s = b->layout[i];
+        if (s.n_e % nodes != 0) return fail(CP3H_ERR_ARGUMENT, "set energies are not a whole number of bins");
+        CP3H_HIP(cp3::launch_bin_average(b->d_out + s.out_off, s.n_c, s.n_e, nodes, weights, out_device + off,
+                                         b->stream));
+        off += std::size_t(9) * s.n_c * (s.n_e / nodes);
+    }
+    CP3H_HIP(hipStreamSynchronize(b->stream));
+    return CP3H_OK;
+}
+
 int cp3h_batch_device_results(cp3h_batch* b, int set, const float** ptr) {
     if (int rc = check(b, set)) return rc;
     if (!ptr) return fail(CP3H_ERR_ARGUMENT, "null output pointer");

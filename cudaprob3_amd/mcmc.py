@@ -48,6 +48,7 @@ class Chain:
         self._lib = _lib.load()
         n = 9 * len(grid.cosines) * len(grid.energies)
         self.data = torch.empty(n, dtype=torch.float32, device=f"cuda:{device}")
+        torch.cuda.current_stream(self.data.device).synchronize()   # written by the propagator's stream
         self._evaluate(true_params)
         _lib.check(self._lib.cp3h_copy_results_device(p._handle(), ctypes.c_void_p(self.data.data_ptr()), n))
         self.params = np.array(true_params, dtype=np.float64)

@@ -1,0 +1,278 @@
+"""Learned surrogate of energy-bin-averaged oscillation probabilities, with the exact kernel as
+its data source, its fallback and the judge of its error.
+
+* Network: 9 -> 256 x 5 (GELU) -> 9, BF16 weights, FP32 accumulation, trained and evaluated
+  by the library's matrix-core kernels (cp3h_mlp_*).  Inputs: cosZ, log10 E, theta12,
+  theta13, theta23, deltaCP, dm21^2, dm32^2, nubar flag; outputs the nine P(in -> out).
+* Targets: each probability averaged over a +-2.5 % energy bin by 16-point Gauss-Legendre
+  quadrature, computed by the exact mixed-precision kernel (batched, cp3h_batch_*) on the
+  12-shell jittered Earth.
+* Fallback: an error map over (cosZ, log10 E) cells, filled with the largest error seen on
+  held-out parameter sets; bins in cells whose bound exceeds the tolerance are recomputed by
+  the exact binned kernel.
+
+PyTorch supplies device memory for the datasets and a few tensor reshapes; the arithmetic is
+the library's.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from . import workloads as W
+from .batch import BatchPropagator, SetSpec
+
+# feature ranges (mapped to [-1, 1]): cosZ, log10 E/GeV, 6 oscillation parameters, flag
+LOG10E_RANGE = (math.log10(0.05), 3.0)
+FEATURE_LO = np.array([-1.0, LOG10E_RANGE[0], 0.55, 0.14, 0.65, 0.0, 6.8e-5, -2.7e-3, 0.0], np.float32)
+FEATURE_HI = np.array([1.0, LOG10E_RANGE[1], 0.62, 0.16, 0.92, 2 * np.pi, 8.0e-5, 2.7e-3, 1.0], np.float32)
+
+GL_NODES, GL_WEIGHTS = np.polynomial.legendre.leggauss(16)
+BIN_HALF_WIDTH = 0.025
+
+
+def gl_energies(centers):
+    """16 Gauss-Legendre nodes of each +-2.5 % bin, bin-major, and the averaging weights."""
+    centers = np.asarray(centers, np.float64)
+    E = (centers[:, None] * (1.0 + BIN_HALF_WIDTH * GL_NODES[None, :])).reshape(-1)
+    return E, (GL_WEIGHTS / 2.0).astype(np.float32)
+
+
+def param_count() -> int:
+    n = ctypes.c_size_t()
+    _lib.check(_lib.load().cp3h_mlp_param_count(ctypes.byref(n)))
+    return n.value
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-normal weights for the GELU layers, small output layer, zero biases."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    shapes = [(256, 9), (256, 256), (256, 256), (256, 256), (256, 256), (9, 256)]
+    parts = []
+    for i, (n, k) in enumerate(shapes):
+        std = math.sqrt(2.0 / k) if i < len(shapes) - 1 else 0.1 / math.sqrt(k)
+        parts.append(rng.normal(0.0, std, n * k))
+        parts.append(np.zeros(n))
+    return np.concatenate(parts).astype(np.float32)
+
+
+class SurrogateNet:
+    def __init__(self, max_batch: int = 8192, training: bool = True, device: int = 0, seed: int = 0):
+        import torch
+        self.torch = torch
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.cp3h_mlp_create(device, max_batch, int(training), ctypes.byref(h)))
+        self._h = h
+        self.max_batch = max_batch
+        self.device = torch.device(f"cuda:{device}")
+        shift = ((FEATURE_LO + FEATURE_HI) / 2).astype(np.float32)
+        scale = (2.0 / (FEATURE_HI - FEATURE_LO)).astype(np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        _lib.check(self._lib.cp3h_mlp_set_scaling(self._h, shift.ctypes.data_as(fp), scale.ctypes.data_as(fp)))
+        self.shift, self.scale = shift, scale
+        # the network runs on a stream torch knows, so the two can be ordered on the device
+        self.stream = torch.cuda.Stream(device=self.device)
+        self.set_stream(self.stream.cuda_stream)
+        self.set_params(init_params(seed))
+
+    def _after_torch(self):
+        """Order the network's next launches after torch's pending work (inputs)."""
+        self.stream.wait_stream(self.torch.cuda.current_stream(self.device))
+
+    def _before_torch(self):
+        """Order torch's next work after the network's launches (outputs)."""
+        self.torch.cuda.current_stream(self.device).wait_stream(self.stream)
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        _lib.check(self._lib.cp3h_mlp_set_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.size))
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(param_count(), np.float32)
+        _lib.check(self._lib.cp3h_mlp_get_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.size))
+        return p
+
+    def set_stream(self, stream_ptr):
+        _lib.check(self._lib.cp3h_mlp_set_stream(self._h, ctypes.c_void_p(stream_ptr or None)))
+
+    def forward(self, x, out=None):
+        """x: cuda float32 [n][9] -> [n][9] (asynchronous on the network's stream)."""
+        x = x.contiguous()
+        if out is None:
+            out = self.torch.empty((x.shape[0], 9), dtype=self.torch.float32, device=x.device)
+        self._after_torch()
+        _lib.check(self._lib.cp3h_mlp_forward(self._h, ctypes.c_void_p(x.data_ptr()), x.shape[0],
+                                              ctypes.c_void_p(out.data_ptr())))
+        self._before_torch()
+        return out
+
+    def sync(self):
+        _lib.check(self._lib.cp3h_mlp_sync(self._h))
+
+    def gradients(self, x, t):
+        g = np.empty(param_count(), np.float32)
+        loss = ctypes.c_float()
+        self._after_torch()
+        _lib.check(self._lib.cp3h_mlp_gradients(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(t.data_ptr()),
+                                                x.shape[0], g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), g.size,
+                                                ctypes.byref(loss)))
+        return g, loss.value
+
+    def train_step(self, x, t, lr: float, want_loss: bool = False):
+        loss = ctypes.c_float()
+        self._after_torch()
+        _lib.check(self._lib.cp3h_mlp_train_step(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(t.data_ptr()),
+                                                 x.shape[0], ctypes.c_float(lr),
+                                                 ctypes.byref(loss) if want_loss else None))
+        self._before_torch()       # the minibatch tensors may be freed and reused by torch
+        return loss.value if want_loss else None
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.cp3h_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# --- data from the exact kernel ---------------------------------------------------------------
+
+def features(torch, cos, log10E, params, antinu, device):
+    """[n_cos * n_bins][9] rows (cosine-major) for one hypothesis."""
+    nc, nb = len(cos), len(log10E)
+    X = torch.empty((nc, nb, 9), dtype=torch.float32, device=device)
+    X[:, :, 0] = torch.as_tensor(np.asarray(cos, np.float32), device=device)[:, None]
+    X[:, :, 1] = torch.as_tensor(np.asarray(log10E, np.float32), device=device)[None, :]
+    for k in range(6):
+        X[:, :, 2 + k] = float(params[k])
+    X[:, :, 8] = float(antinu)
+    return X.reshape(nc * nb, 9)
+
+
+@dataclass
+class BinnedSets:
+    """Exact bin-averaged probabilities of many hypotheses: X [n][9], T [n][9] on the GPU."""
+    X: object
+    T: object
+    specs: list
+
+
+def exact_binned(specs_bins, device=0):
+    """specs_bins: list of (cosines, bin centres, params, antinu) on the jittered 12-shell
+    Earth.  Returns per-set device tensors [9][n_cos][n_bins] of GL bin averages."""
+    import torch
+    radii, rhos = W.jitter12_model()
+    specs, weights = [], None
+    for cos, centers, p, nu in specs_bins:
+        E, weights = gl_energies(centers)
+        specs.append(SetSpec(np.asarray(cos, np.float64), E, radii, rhos, tuple(p), int(nu)))
+    b = BatchPropagator(specs, device=device, graph=False)
+    b.compute()
+    sizes = [9 * len(s.cosines) * (len(s.energies) // 16) for s in specs]
+    out = torch.empty(sum(sizes), dtype=torch.float32, device=f"cuda:{device}")
+    torch.cuda.current_stream(out.device).synchronize()   # the batch's stream writes into torch memory
+    wp = weights.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    _lib.check(b._lib.cp3h_batch_bin_average_f32(b._h, 0, len(specs), 16, wp, ctypes.c_void_p(out.data_ptr())))
+    b.close()
+    res, off = [], 0
+    for (cos, centers, _, _), n in zip(specs_bins, sizes):
+        res.append(out[off:off + n].view(9, len(cos), len(centers)))
+        off += n
+    return res
+
+
+def random_hypothesis(rng):
+    p = list(W.random_params(rng))
+    return p, int(rng.random() < 0.5)
+
+
+def make_dataset(n_sets: int, n_cos: int, n_bins: int, seed: int, device=0, chunk: int = 256) -> BinnedSets:
+    """Random hypotheses (parameter ranges of the scan workload, either ordering, nu or
+    nubar), each on n_cos random cosines x n_bins random log-uniform bin centres in
+    [0.05, 1000] GeV; exact GL-binned targets."""
+    import torch
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dev = torch.device(f"cuda:{device}")
+    Xs, Ts, specs = [], [], []
+    for c0 in range(0, n_sets, chunk):
+        batch = []
+        for _ in range(min(chunk, n_sets - c0)):
+            p, nu = random_hypothesis(rng)
+            cos = np.sort(rng.uniform(-1.0, 1.0, n_cos))
+            l10 = np.sort(rng.uniform(LOG10E_RANGE[0], LOG10E_RANGE[1], n_bins))
+            batch.append((cos, 10.0 ** l10, p, nu))
+        for (cos, centers, p, nu), P in zip(batch, exact_binned(batch, device)):
+            Xs.append(features(torch, cos, np.log10(centers), p, nu, dev))
+            Ts.append(P.permute(1, 2, 0).reshape(-1, 9))
+            specs.append((p, nu))
+    return BinnedSets(torch.cat(Xs), torch.cat(Ts), specs)
+
+
+def train(net: SurrogateNet, data: BinnedSets, epochs: int, batch: int = 8192, lr_max: float = 2e-3,
+          lr_min: float = 2e-5, seed: int = 0, log=None):
+    """Adam with a cosine learning-rate decay; minibatches drawn without replacement."""
+    torch = net.torch
+    n = data.X.shape[0]
+    steps_per_epoch = n // batch
+    total = epochs * steps_per_epoch
+    gen = torch.Generator(device=data.X.device)
+    gen.manual_seed(seed)
+    step = 0
+    history = []
+    for ep in range(epochs):
+        perm = torch.randperm(n, device=data.X.device, generator=gen)
+        for s in range(steps_per_epoch):
+            idx = perm[s * batch:(s + 1) * batch]
+            x, t = data.X[idx].contiguous(), data.T[idx].contiguous()
+            lr = lr_min + 0.5 * (lr_max - lr_min) * (1 + math.cos(math.pi * step / max(total, 1)))
+            want = s == steps_per_epoch - 1
+            loss = net.train_step(x, t, lr, want_loss=want)
+            if want:
+                history.append(loss)
+                if log:
+                    log(f"epoch {ep}: loss {loss:.3e} lr {lr:.2e}")
+            step += 1
+    net.sync()
+    return history
+
+
+# --- error map and fallback ---------------------------------------------------------------------
+
+@dataclass
+class ErrorMap:
+    n_cos: int
+    n_e: int
+    max_err: np.ndarray      # [n_cos][n_e] largest |dP| seen per cell (all nine channels)
+
+    def cell(self, cos, log10E):
+        ic = np.clip(((np.asarray(cos) + 1.0) / 2.0 * self.n_cos).astype(int), 0, self.n_cos - 1)
+        ie = np.clip(((np.asarray(log10E) - LOG10E_RANGE[0]) / (LOG10E_RANGE[1] - LOG10E_RANGE[0]) * self.n_e)
+                     .astype(int), 0, self.n_e - 1)
+        return ic, ie
+
+    def fallback_mask(self, tol: float, safety: float = 2.0) -> np.ndarray:
+        return self.max_err * safety > tol
+
+
+def build_error_map(net: SurrogateNet, data: BinnedSets, n_cos=32, n_e=32) -> ErrorMap:
+    torch = net.torch
+    Y = net.forward(data.X)
+    net.sync()
+    err = (Y - data.T).abs().amax(dim=1)
+    cos = data.X[:, 0]
+    l10 = data.X[:, 1]
+    ic = ((cos + 1.0) / 2.0 * n_cos).long().clamp(0, n_cos - 1)
+    ie = ((l10 - LOG10E_RANGE[0]) / (LOG10E_RANGE[1] - LOG10E_RANGE[0]) * n_e).long().clamp(0, n_e - 1)
+    cell = ic * n_e + ie
+    m = torch.zeros(n_cos * n_e, dtype=torch.float32, device=err.device)
+    m.scatter_reduce_(0, cell, err, reduce="amax")
+    return ErrorMap(n_cos, n_e, m.view(n_cos, n_e).cpu().numpy())

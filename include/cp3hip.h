@@ -172,6 +172,43 @@ int cp3h_batch_last_ms(cp3h_batch* b, float* total_ms, float* pairs_ms);
 int cp3h_batch_copy_results_f32(cp3h_batch* b, int set, float* dst, size_t count);
 int cp3h_batch_device_results(cp3h_batch* b, int set, const float** ptr);
 
+/* Energy-bin averages of sets first .. first+count-1 whose energy lists are `nodes`
+   quadrature nodes per bin (bin-major): out_device receives, set after set, the SoA array
+   [9][n_cosines][n_energies / nodes] of sum_g weights[g] * P(node g).  Orders after the last
+   computation on the batch's stream and waits. */
+int cp3h_batch_bin_average_f32(cp3h_batch* b, int first, int count, int nodes, const float* weights,
+                               float* out_device);
+
+/* ---- Surrogate network --------------------------------------------------------------------
+ * No counterpart in the reference.  A 9 -> 256 x 5 (GELU) -> 9 MLP with BF16 weights and FP32
+ * accumulation on the matrix cores, approximating energy-bin-averaged probabilities from
+ * (cosZ, log10 E, theta12, theta13, theta23, deltaCP, dm21^2, dm32^2, nubar flag).  Inputs
+ * are device float arrays [n][9]; outputs device float [n][9] in the order t = in*3 + out.
+ * Parameters (fp32, cp3h_mlp_param_count of them) are layer by layer: W_l row-major
+ * [outputs][inputs], then b_l. */
+typedef struct cp3h_mlp cp3h_mlp;
+
+int cp3h_mlp_param_count(size_t* n);
+/* max_batch: rows per internal launch (multiple of 256); training: also keep the backward
+   buffers, gradients and Adam state. */
+int cp3h_mlp_create(int device, int max_batch, int training, cp3h_mlp** out);
+int cp3h_mlp_destroy(cp3h_mlp* h);
+int cp3h_mlp_set_stream(cp3h_mlp* h, void* stream);
+/* inputs are mapped to (x - shift) * scale before the first layer */
+int cp3h_mlp_set_scaling(cp3h_mlp* h, const float* shift9, const float* scale9);
+/* sets the fp32 parameters (and resets the optimizer state) / reads them back */
+int cp3h_mlp_set_params(cp3h_mlp* h, const float* params, size_t n);
+int cp3h_mlp_get_params(cp3h_mlp* h, float* params, size_t n);
+/* inference of n rows (asynchronous on the network's stream) */
+int cp3h_mlp_forward(cp3h_mlp* h, const float* x_device, long long n, float* y_device);
+/* gradients of the mean squared error over the 9 outputs of one batch (M a multiple of 256,
+   <= max_batch), copied to host memory in the parameter layout, and the loss */
+int cp3h_mlp_gradients(cp3h_mlp* h, const float* x_device, const float* t_device, int M, float* grads, size_t n,
+                       float* loss);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on one batch; loss may be NULL (no wait) */
+int cp3h_mlp_train_step(cp3h_mlp* h, const float* x_device, const float* t_device, int M, float lr, float* loss);
+int cp3h_mlp_sync(cp3h_mlp* h);
+
 #ifdef __cplusplus
 }
 #endif
