@@ -17,6 +17,7 @@ CP3H_FP64_PLAIN = 0
 CP3H_FP64 = 1
 CP3H_MIXED = 2
 PRECISIONS = {"fp64_plain": CP3H_FP64_PLAIN, "fp64": CP3H_FP64, "mixed": CP3H_MIXED}
+MLP_FORWARD_KERNELS = {"layered": 0, "fused4": 1, "fused8": 2}
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -78,6 +79,7 @@ SIGNATURES = {
     "cp3h_mlp_set_params": (ctypes.c_int, [_handle, _c_float_p, ctypes.c_size_t]),
     "cp3h_mlp_get_params": (ctypes.c_int, [_handle, _c_float_p, ctypes.c_size_t]),
     "cp3h_mlp_forward": (ctypes.c_int, [_handle, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p]),
+    "cp3h_mlp_set_forward_kernel": (ctypes.c_int, [_handle, ctypes.c_int]),
     "cp3h_mlp_gradients": (ctypes.c_int, [_handle, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_float_p,
                                           ctypes.c_size_t, _c_float_p]),
     "cp3h_mlp_train_step": (ctypes.c_int, [_handle, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float,

@@ -34,6 +34,10 @@ constexpr LayerShape kShape[cp3::kMlpLayers] = {
 
 constexpr int kSplit = 8;   // split-K parts of the weight-gradient GEMMs
 
+// bf16 forward weights of layer l: [N][wb_ld(l)], each row's K values at 0 and at wb_dup(l)
+constexpr int wb_ld(int l) { return l == 0 ? cp3::kMlpInPad : 2 * kShape[l].K; }
+constexpr int wb_dup(int l) { return l == 0 ? cp3::kMlpInLo : kShape[l].K; }
+
 std::size_t param_count() {
     std::size_t n = 0;
     for (const auto& s : kShape) n += std::size_t(s.n_valid) * s.k_valid + s.n_valid;
@@ -66,6 +70,7 @@ struct cp3h_mlp {
     float* loss_host = nullptr;
     cp3::MlpScaling scaling{};
     int adam_t = 0;
+    int forward_kernel = CP3H_MLP_FUSED8;
     std::vector<void*> allocations;
 };
 
@@ -89,7 +94,7 @@ int refresh_bf16(cp3h_mlp* h) {
     const cp3::AdamStep none{0.0f, 0.0f, 0.0f, 1.0f, 1.0f, 1.0f};
     for (int l = 0; l < cp3::kMlpLayers; l++)
         CP3H_HIP(cp3::launch_adam_layer(h->w[l], nullptr, nullptr, nullptr, kShape[l].N, kShape[l].K, 0, 0, none,
-                                        h->wb[l], h->training ? h->wtb[l] : nullptr, h->stream));
+                                        h->wb[l], wb_ld(l), wb_dup(l), h->training ? h->wtb[l] : nullptr, h->stream));
     return CP3H_OK;
 }
 
@@ -101,16 +106,19 @@ int forward(cp3h_mlp* h, const float* x, int M, const float* targets, float* y_o
     const __bf16* prev = h->x0;
     for (int l = 0; l < cp3::kMlpHiddenLayers; l++) {
         const LayerShape& s = kShape[l];
+        const int kk = wb_ld(l);   // hi and lo terms of the previous activations
         // inference ping-pongs between two activation buffers
         __bf16* out = train ? h->y[l] : h->y[l % 2];
-        cp3::GemmDesc d{prev, s.K, h->wb[l], s.K, M, s.N, s.K, 1, cp3::kEpiBiasGelu, h->b[l], nullptr, 0,
-                        train ? h->z[l] : nullptr, s.N, out, s.N, train ? h->yt[l] : nullptr, M, s.n_valid};
+        cp3::GemmDesc d{prev, kk, h->wb[l], kk, M, s.N, kk, 1, cp3::kEpiBiasGelu, h->b[l], nullptr, 0,
+                        train ? h->z[l] : nullptr, s.N, out, cp3::kMlpActLd, train ? h->yt[l] : nullptr, M,
+                        s.n_valid, cp3::kMlpHidden};
         CP3H_HIP(cp3::launch_gemm_nt(d, h->stream));
         prev = out;
     }
-    const LayerShape& o = kShape[cp3::kMlpLayers - 1];
-    cp3::GemmDesc d{prev, o.K, h->wb[cp3::kMlpLayers - 1], o.K, M, o.N, o.K, 1, cp3::kEpiBias,
-                    h->b[cp3::kMlpLayers - 1], nullptr, 0, h->zout, o.N, nullptr, 0, nullptr, 0, o.n_valid};
+    constexpr int lo = cp3::kMlpLayers - 1;
+    const LayerShape& o = kShape[lo];
+    cp3::GemmDesc d{prev, wb_ld(lo), h->wb[lo], wb_ld(lo), M, o.N, wb_ld(lo), 1, cp3::kEpiBias, h->b[lo], nullptr,
+                    0, h->zout, o.N, nullptr, 0, nullptr, 0, o.n_valid, 0};
     CP3H_HIP(cp3::launch_gemm_nt(d, h->stream));
     CP3H_HIP(cp3::launch_output_head(h->zout, M, targets, y_out, train ? h->dz[0] : nullptr,
                                      train ? h->dzt[0] : nullptr, train ? h->loss_part : nullptr, h->stream));
@@ -126,14 +134,14 @@ int backward(cp3h_mlp* h, int M) {
         const __bf16* prev_t = l == 0 ? h->x0t : h->yt[l - 1];
         // dW = dZ^T Y_{l-1}: A = dZ^T [N][M], B = Y_{l-1}^T [K][M], reduction over M in kSplit parts
         cp3::GemmDesc gwd{h->dzt[cur], M, prev_t, M, s.N, s.K, M, kSplit, cp3::kEpiPartial, nullptr, nullptr, 0,
-                          h->partials, s.K, nullptr, 0, nullptr, 0, s.N};
+                          h->partials, s.K, nullptr, 0, nullptr, 0, s.N, 0};
         CP3H_HIP(cp3::launch_gemm_nt(gwd, h->stream));
         CP3H_HIP(cp3::launch_sum_parts(h->partials, kSplit, s.N * s.K, h->gw[l], h->stream));
         CP3H_HIP(cp3::launch_row_sums(h->dzt[cur], s.N, M, h->gb[l], h->stream));
         if (l > 0) {
             // dY_{l-1} = dZ W: A = dZ [M][N], B = W^T [K][N]; epilogue applies gelu'(Z_{l-1})
             cp3::GemmDesc gx{h->dz[cur], s.N, h->wtb[l], s.N, M, s.K, s.N, 1, cp3::kEpiGeluGrad, nullptr,
-                             h->z[l - 1], s.K, nullptr, 0, h->dz[1 - cur], s.K, h->dzt[1 - cur], M, s.k_valid};
+                             h->z[l - 1], s.K, nullptr, 0, h->dz[1 - cur], s.K, h->dzt[1 - cur], M, s.k_valid, 0};
             CP3H_HIP(cp3::launch_gemm_nt(gx, h->stream));
             cur = 1 - cur;
         }
@@ -187,7 +195,7 @@ int cp3h_mlp_create(int device, int max_batch, int training, cp3h_mlp** out) {
             const std::size_t nk = std::size_t(kShape[l].N) * kShape[l].K;
             if (int e = dev_alloc(h, &h->w[l], nk)) return e;
             if (int e = dev_alloc(h, &h->b[l], kShape[l].N)) return e;
-            if (int e = dev_alloc(h, &h->wb[l], nk)) return e;
+            if (int e = dev_alloc(h, &h->wb[l], std::size_t(kShape[l].N) * wb_ld(l))) return e;
             if (h->training) {
                 if (int e = dev_alloc(h, &h->wtb[l], nk)) return e;
                 if (int e = dev_alloc(h, &h->m1w[l], nk)) return e;
@@ -205,7 +213,7 @@ int cp3h_mlp_create(int device, int max_batch, int training, cp3h_mlp** out) {
             if (int e = dev_alloc(h, &h->x0t, M * cp3::kMlpInPad)) return e;
             for (int l = 0; l < cp3::kMlpHiddenLayers; l++) {
                 if (int e = dev_alloc(h, &h->z[l], M * cp3::kMlpHidden)) return e;
-                if (int e = dev_alloc(h, &h->y[l], M * cp3::kMlpHidden)) return e;
+                if (int e = dev_alloc(h, &h->y[l], M * cp3::kMlpActLd)) return e;
                 if (int e = dev_alloc(h, &h->yt[l], M * cp3::kMlpHidden)) return e;
             }
             for (int k = 0; k < 2; k++) {
@@ -215,7 +223,7 @@ int cp3h_mlp_create(int device, int max_batch, int training, cp3h_mlp** out) {
             if (int e = dev_alloc(h, &h->loss_part, std::size_t(cp3::output_head_blocks(max_batch)))) return e;
         } else {
             for (int l = 0; l < 2; l++)
-                if (int e = dev_alloc(h, &h->y[l], M * cp3::kMlpHidden)) return e;
+                if (int e = dev_alloc(h, &h->y[l], M * cp3::kMlpActLd)) return e;
         }
         if (int e = dev_alloc(h, &h->loss_dev, 1)) return e;
         CP3H_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->loss_host), sizeof(float)));
@@ -311,10 +319,23 @@ int cp3h_mlp_forward(cp3h_mlp* h, const float* x_device, long long n, float* y_d
     if (int rc = check(h)) return rc;
     if (n < 0 || (n > 0 && (!x_device || !y_device))) return fail(CP3H_ERR_ARGUMENT, "bad forward arguments");
     CP3H_HIP(hipSetDevice(h->device));
+    if (h->forward_kernel != CP3H_MLP_LAYERED) {
+        CP3H_HIP(cp3::launch_mlp_forward_fused(x_device, n, y_device, h->wb, h->b, h->scaling,
+                                               h->forward_kernel == CP3H_MLP_FUSED8 ? 2 : 1, h->stream));
+        return CP3H_OK;
+    }
     for (long long r0 = 0; r0 < n; r0 += h->max_batch) {
         const int M = int(std::min<long long>(h->max_batch, n - r0));
         if (int rc = forward(h, x_device + r0 * cp3::kMlpIn, M, nullptr, y_device + r0 * cp3::kMlpOut)) return rc;
     }
+    return CP3H_OK;
+}
+
+int cp3h_mlp_set_forward_kernel(cp3h_mlp* h, int kernel) {
+    if (int rc = check(h)) return rc;
+    if (kernel != CP3H_MLP_LAYERED && kernel != CP3H_MLP_FUSED4 && kernel != CP3H_MLP_FUSED8)
+        return fail(CP3H_ERR_ARGUMENT, "unknown forward kernel");
+    h->forward_kernel = kernel;
     return CP3H_OK;
 }
 
@@ -356,7 +377,7 @@ int cp3h_mlp_train_step(cp3h_mlp* h, const float* x_device, const float* t_devic
     for (int l = 0; l < cp3::kMlpLayers; l++) {
         const LayerShape& s = kShape[l];
         CP3H_HIP(cp3::launch_adam_layer(h->w[l], h->m1w[l], h->m2w[l], h->gw[l], s.N, s.K, s.n_valid, s.k_valid, a,
-                                        h->wb[l], h->wtb[l], h->stream));
+                                        h->wb[l], wb_ld(l), wb_dup(l), h->wtb[l], h->stream));
         CP3H_HIP(cp3::launch_adam_vec(h->b[l], h->m1b[l], h->m2b[l], h->gb[l], s.N, s.n_valid, a, h->stream));
     }
     if (loss) return loss_to_host(h, M, loss);

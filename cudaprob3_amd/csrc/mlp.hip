@@ -17,22 +17,18 @@
 //
 // MFMA operand layout (16x16x32, lane l): A row l % 16, k = 8 (l / 16) .. +7; B column
 // l % 16, same k; C column l % 16, rows 4 (l / 16) .. +3.
+//
+// Forward products take two-term activations (hi + lo, see mlp.hpp) as a K twice as long
+// against weights stored twice per row; the backward pass uses the hi terms only (Y^T, dZ).
 #include "common.hpp"
 #include "mlp.hpp"
+#include "mlp_device.hpp"
 
 namespace cp3 {
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
 constexpr int kKChunk = 256;
 constexpr int kPad = 8;
-
-__device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
-__device__ inline float gelu_grad(float x) {
-    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * __expf(-0.5f * x * x);
-}
 
 struct GemmArgs {
     const __bf16* A; int lda;      // [M][K]
@@ -46,6 +42,7 @@ struct GemmArgs {
     __bf16* yout; int ldy;         // bf16 output [M][N]
     __bf16* ytout; int ldyt;       // bf16 transposed output [N][M]
     int n_valid;                   // columns >= n_valid are written as zero (padding)
+    int lo_off;                    // kEpiBiasGelu: lo terms at yout[row][lo_off + col] (0: none)
 };
 
 template <int TM, int TN, int kEpi>
@@ -128,8 +125,10 @@ __global__ __launch_bounds__(256) void gemm_nt(GemmArgs g) {
                 if (col >= g.n_valid) v = 0.0f;
                 if (kEpi == kEpiBiasGelu) {
                     if (g.zout) g.zout[std::size_t(row) * g.ldzo + col] = v;
-                    const __bf16 y = __bf16(gelu(v));
+                    const float a = gelu(v);
+                    const __bf16 y = __bf16(a);
                     g.yout[std::size_t(row) * g.ldy + col] = y;
+                    if (g.lo_off) g.yout[std::size_t(row) * g.ldy + g.lo_off + col] = __bf16(a - float(y));
                     if (g.ytout) g.ytout[std::size_t(col) * g.ldyt + row] = y;
                 } else if (kEpi == kEpiBias) {
                     g.zout[std::size_t(row) * g.ldzo + col] = v;
@@ -144,18 +143,21 @@ __global__ __launch_bounds__(256) void gemm_nt(GemmArgs g) {
 
 // --- small kernels ---------------------------------------------------------------------------
 
-// x[M][9] fp32 raw inputs -> (x - shift) * scale as bf16 [M][32] (K padded with zeros), and
-// its transpose [32][M] for the first layer's weight gradient.
+// x[M][9] fp32 raw inputs -> v = (x - shift) * scale as two-term bf16 [M][32]: hi in columns
+// 0..8, lo in kMlpInLo..kMlpInLo+8, zeros elsewhere; and the hi terms transposed, [32][M], for
+// the first layer's weight gradient.
 __global__ __launch_bounds__(256) void featurize(const float* __restrict__ x, int M, MlpScaling s,
                                                  __bf16* __restrict__ f, __bf16* __restrict__ ft) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= M * kMlpInPad) return;
     const int m = t / kMlpInPad, c = t % kMlpInPad;
+    const int src = c < kMlpIn ? c : c - kMlpInLo;
     float v = 0.0f;
-    if (c < kMlpIn) v = (x[std::size_t(m) * kMlpIn + c] - s.shift[c]) * s.scale[c];
-    const __bf16 b = __bf16(v);
+    if (src >= 0 && src < kMlpIn) v = (x[std::size_t(m) * kMlpIn + src] - s.shift[src]) * s.scale[src];
+    const __bf16 hi = __bf16(v);
+    const __bf16 b = c < kMlpIn ? hi : __bf16(v - float(hi));
     f[t] = b;
-    if (ft) ft[std::size_t(c) * M + m] = b;
+    if (ft) ft[std::size_t(c) * M + m] = c < kMlpIn ? hi : __bf16(0.0f);
 }
 
 // output layer: y = z[:, :9] (z has kMlpOutPad columns); with targets, the loss partial per workgroup
@@ -229,10 +231,12 @@ __global__ __launch_bounds__(256) void sum_floats(const float* __restrict__ p, i
 }
 
 // Adam on one layer's padded fp32 weights [N][K] (+ bias [N]); gradients of padding are zero.
-// Writes the bf16 copies W [N][K] and W^T [K][N].
+// Writes the bf16 copies: W [N][ld_wb] with each row's K values at column 0 and again at
+// column `dup` (the weights seen by the lo terms), and W^T [K][N].
 __global__ __launch_bounds__(256) void adam_layer(float* __restrict__ w, float* __restrict__ m1, float* __restrict__ m2,
                                                   const float* __restrict__ g, int N, int K, int n_valid, int k_valid,
-                                                  AdamStep a, __bf16* __restrict__ wb, __bf16* __restrict__ wtb) {
+                                                  AdamStep a, __bf16* __restrict__ wb, int ld_wb, int dup,
+                                                  __bf16* __restrict__ wtb) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N * K) return;
     const int n = i / K, k = i % K;
@@ -247,7 +251,10 @@ __global__ __launch_bounds__(256) void adam_layer(float* __restrict__ w, float* 
         w[i] = wi;
     }
     const __bf16 b = __bf16(wi);
-    if (wb) wb[i] = b;
+    if (wb) {
+        if (k < dup) wb[std::size_t(n) * ld_wb + k] = b;
+        if (dup + k < ld_wb) wb[std::size_t(n) * ld_wb + dup + k] = b;
+    }
     if (wtb) wtb[std::size_t(k) * N + n] = b;
 }
 
@@ -271,7 +278,7 @@ hipError_t launch_gemm_nt(const GemmDesc& d, hipStream_t s) {
     if (d.K % 32 != 0 || d.N % 16 != 0 || d.k_split < 1 || (d.K / d.k_split) % 32 != 0 || d.K % d.k_split != 0)
         return hipErrorInvalidValue;
     GemmArgs g{d.A, d.lda, d.B, d.ldb, d.M, d.N, d.K, d.k_split, d.bias, d.zprev, d.ldz, d.zout, d.ldzo,
-               d.yout, d.ldy, d.ytout, d.ldyt, d.n_valid};
+               d.yout, d.ldy, d.ytout, d.ldyt, d.n_valid, d.lo_off};
     const bool small = d.N <= 32 || d.M <= 4096;
 #define CP3_GEMM(TM, TN, EPI)                                                                              \
     hipLaunchKernelGGL((gemm_nt<TM, TN, EPI>), dim3((d.M + 32 * TM - 1) / (32 * TM), (d.N + 32 * TN - 1) / (32 * TN), \
@@ -319,9 +326,10 @@ hipError_t launch_sum_floats(const float* p, int n, float* out, hipStream_t s) {
 }
 
 hipError_t launch_adam_layer(float* w, float* m1, float* m2, const float* g, int N, int K, int n_valid, int k_valid,
-                             const AdamStep& a, __bf16* wb, __bf16* wtb, hipStream_t s) {
+                             const AdamStep& a, __bf16* wb, int ld_wb, int dup, __bf16* wtb, hipStream_t s) {
+    if (wb && (dup > ld_wb || ld_wb > 2 * K || dup < 1)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(adam_layer, dim3(blocks_for((long long)N * K)), dim3(256), 0, s, w, m1, m2, g, N, K, n_valid,
-                       k_valid, a, wb, wtb);
+                       k_valid, a, wb, ld_wb, dup, wtb);
     return hipGetLastError();
 }
 

@@ -16,6 +16,16 @@ constexpr int kMlpHidden = 256, kMlpHiddenLayers = 5;
 constexpr int kMlpOut = 9, kMlpOutPad = 32;
 constexpr int kMlpLayers = kMlpHiddenLayers + 1;
 
+// Two-term activations.  The forward pass carries every activation (and every scaled input)
+// as a pair of bf16 values, hi = bf16(a) and lo = bf16(a - hi), whose sum holds a to about
+// 16 significant bits; a product with a bf16 weight row then takes two MFMA k-steps against
+// the same weights.  The layouts: scaled inputs [M][kMlpInPad] with hi in columns 0..8 and lo
+// in columns kMlpInLo..kMlpInLo+8; hidden activations [M][kMlpActLd], hi in 0..255, lo in
+// 256..511; the bf16 weights repeat each row's K values at the lo offset.  Single-term bf16
+// activations alone put errors of order 1e-3..1e-2 on the outputs.
+constexpr int kMlpInLo = kMlpIn;
+constexpr int kMlpActLd = 2 * kMlpHidden;
+
 // Epilogues of the NT GEMM.
 enum GemmEpilogue {
     kEpiBiasGelu = 0,   // Z = acc + b -> zout (fp32, optional); Y = gelu(Z) -> yout, ytout (bf16)
@@ -35,6 +45,7 @@ struct GemmDesc {
     __bf16* yout; int ldy;
     __bf16* ytout; int ldyt;
     int n_valid;
+    int lo_off;        // kEpiBiasGelu: column offset of the lo terms in yout (0: hi only)
 };
 
 struct MlpScaling {
@@ -54,8 +65,12 @@ int output_head_blocks(int M);
 hipError_t launch_row_sums(const __bf16* a, int rows, int M, float* out, hipStream_t s);
 hipError_t launch_sum_parts(const float* p, int parts, int n, float* out, hipStream_t s);
 hipError_t launch_sum_floats(const float* p, int n, float* out, hipStream_t s);
+// wb: bf16 weights [N][ld_wb], each row's K values at column 0 and again at column `dup`
 hipError_t launch_adam_layer(float* w, float* m1, float* m2, const float* g, int N, int K, int n_valid, int k_valid,
-                             const AdamStep& a, __bf16* wb, __bf16* wtb, hipStream_t s);
+                             const AdamStep& a, __bf16* wb, int ld_wb, int dup, __bf16* wtb, hipStream_t s);
+// all layers of inference in one launch (mlp_infer.hip); row_groups 1 or 2 (4 or 8 wavefronts)
+hipError_t launch_mlp_forward_fused(const float* x, long long n, float* y, const __bf16* const* w,
+                                    const float* const* b, const MlpScaling& sc, int row_groups, hipStream_t s);
 hipError_t launch_adam_vec(float* w, float* m1, float* m2, const float* g, int n, int n_valid, const AdamStep& a,
                            hipStream_t s);
 

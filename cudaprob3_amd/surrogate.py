@@ -1,8 +1,9 @@
 """Learned surrogate of energy-bin-averaged oscillation probabilities, with the exact kernel as
 its data source, its fallback and the judge of its error.
 
-* Network: 9 -> 256 x 5 (GELU) -> 9, BF16 weights, FP32 accumulation, trained and evaluated
-  by the library's matrix-core kernels (cp3h_mlp_*).  Inputs: cosZ, log10 E, theta12,
+* Network: 9 -> 256 x 5 (GELU, tanh form) -> 9, BF16 weights, FP32 accumulation, activations
+  carried as two bf16 terms; trained and evaluated by the library's matrix-core kernels
+  (cp3h_mlp_*; inference runs all layers in one fused launch).  Inputs: cosZ, log10 E, theta12,
   theta13, theta23, deltaCP, dm21^2, dm32^2, nubar flag; outputs the nine P(in -> out).
 * Targets: each probability averaged over a +-2.5 % energy bin by 16-point Gauss-Legendre
   quadrature, computed by the exact mixed-precision kernel (batched, cp3h_batch_*) on the
@@ -96,6 +97,10 @@ class SurrogateNet:
         p = np.empty(param_count(), np.float32)
         _lib.check(self._lib.cp3h_mlp_get_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.size))
         return p
+
+    def set_forward_kernel(self, name: str):
+        """'fused8' (default), 'fused4' or 'layered' (the training path's per-layer GEMMs)."""
+        _lib.check(self._lib.cp3h_mlp_set_forward_kernel(self._h, _lib.MLP_FORWARD_KERNELS[name]))
 
     def set_stream(self, stream_ptr):
         _lib.check(self._lib.cp3h_mlp_set_stream(self._h, ctypes.c_void_p(stream_ptr or None)))
@@ -195,10 +200,11 @@ def random_hypothesis(rng):
     return p, int(rng.random() < 0.5)
 
 
-def make_dataset(n_sets: int, n_cos: int, n_bins: int, seed: int, device=0, chunk: int = 256) -> BinnedSets:
+def make_dataset(n_sets: int, n_cos: int, n_bins: int, seed: int, device=0, chunk: int = 256,
+                 log10e_range=LOG10E_RANGE) -> BinnedSets:
     """Random hypotheses (parameter ranges of the scan workload, either ordering, nu or
     nubar), each on n_cos random cosines x n_bins random log-uniform bin centres in
-    [0.05, 1000] GeV; exact GL-binned targets."""
+    10**log10e_range GeV (default [0.05, 1000]); exact GL-binned targets."""
     import torch
     rng = np.random.Generator(np.random.PCG64(seed))
     dev = torch.device(f"cuda:{device}")
@@ -208,7 +214,7 @@ def make_dataset(n_sets: int, n_cos: int, n_bins: int, seed: int, device=0, chun
         for _ in range(min(chunk, n_sets - c0)):
             p, nu = random_hypothesis(rng)
             cos = np.sort(rng.uniform(-1.0, 1.0, n_cos))
-            l10 = np.sort(rng.uniform(LOG10E_RANGE[0], LOG10E_RANGE[1], n_bins))
+            l10 = np.sort(rng.uniform(log10e_range[0], log10e_range[1], n_bins))
             batch.append((cos, 10.0 ** l10, p, nu))
         for (cos, centers, p, nu), P in zip(batch, exact_binned(batch, device)):
             Xs.append(features(torch, cos, np.log10(centers), p, nu, dev))

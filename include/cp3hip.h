@@ -180,12 +180,13 @@ int cp3h_batch_bin_average_f32(cp3h_batch* b, int first, int count, int nodes, c
                                float* out_device);
 
 /* ---- Surrogate network --------------------------------------------------------------------
- * No counterpart in the reference.  A 9 -> 256 x 5 (GELU) -> 9 MLP with BF16 weights and FP32
- * accumulation on the matrix cores, approximating energy-bin-averaged probabilities from
- * (cosZ, log10 E, theta12, theta13, theta23, deltaCP, dm21^2, dm32^2, nubar flag).  Inputs
- * are device float arrays [n][9]; outputs device float [n][9] in the order t = in*3 + out.
- * Parameters (fp32, cp3h_mlp_param_count of them) are layer by layer: W_l row-major
- * [outputs][inputs], then b_l. */
+ * No counterpart in the reference.  A 9 -> 256 x 5 (GELU, tanh form) -> 9 MLP with BF16
+ * weights and FP32 accumulation on the matrix cores, approximating energy-bin-averaged
+ * probabilities from (cosZ, log10 E, theta12, theta13, theta23, deltaCP, dm21^2, dm32^2,
+ * nubar flag).  Activations enter each product as two bf16 terms (hi + lo, about 16
+ * significant bits).  Inputs are device float arrays [n][9]; outputs device float [n][9] in
+ * the order t = in*3 + out.  Parameters (fp32, cp3h_mlp_param_count of them) are layer by
+ * layer: W_l row-major [outputs][inputs], then b_l. */
 typedef struct cp3h_mlp cp3h_mlp;
 
 int cp3h_mlp_param_count(size_t* n);
@@ -201,6 +202,11 @@ int cp3h_mlp_set_params(cp3h_mlp* h, const float* params, size_t n);
 int cp3h_mlp_get_params(cp3h_mlp* h, float* params, size_t n);
 /* inference of n rows (asynchronous on the network's stream) */
 int cp3h_mlp_forward(cp3h_mlp* h, const float* x_device, long long n, float* y_device);
+/* inference kernel: CP3H_MLP_FUSED4 / CP3H_MLP_FUSED8 run all layers in one launch per 128
+   rows (4 or 8 wavefronts per workgroup; FUSED8 is the default); CP3H_MLP_LAYERED runs the
+   training path's per-layer GEMMs in chunks of max_batch rows. */
+enum { CP3H_MLP_LAYERED = 0, CP3H_MLP_FUSED4 = 1, CP3H_MLP_FUSED8 = 2 };
+int cp3h_mlp_set_forward_kernel(cp3h_mlp* h, int kernel);
 /* gradients of the mean squared error over the 9 outputs of one batch (M a multiple of 256,
    <= max_batch), copied to host memory in the parameter layout, and the loss */
 int cp3h_mlp_gradients(cp3h_mlp* h, const float* x_device, const float* t_device, int M, float* grads, size_t n,

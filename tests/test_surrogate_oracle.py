@@ -27,6 +27,16 @@ def test_gradients_match_finite_differences():
         assert abs(fd - g[i]) <= 1e-7 + 1e-5 * abs(fd), (i, fd, g[i])
 
 
+def test_gelu_is_torch_tanh_gelu():
+    import torch
+    z = np.linspace(-12, 12, 4001)
+    ref = torch.nn.functional.gelu(torch.tensor(z, dtype=torch.float64), approximate="tanh").numpy()
+    assert np.abs(mlp_ref.gelu(z) - ref).max() < 1e-14
+    h = 1e-6
+    fd = (mlp_ref.gelu(z + h) - mlp_ref.gelu(z - h)) / (2 * h)
+    assert np.abs(mlp_ref.gelu_grad(z) - fd).max() < 1e-8
+
+
 def test_bf16_rounding():
     v = np.array([1.0, 1.00390625, 1.005859375, -3.14159265, 65504.0, 1e-30], np.float32)
     r = mlp_ref.bf16_round(v)
@@ -35,6 +45,30 @@ def test_bf16_rounding():
     assert abs(r[3] + 3.140625) < 1e-12
     bits = r.astype(np.float32).view(np.uint32)
     assert np.all(bits & 0xFFFF == 0)
+
+
+def test_two_term_split():
+    rng = np.random.default_rng(4)
+    a = rng.normal(0, 3, 10000).astype(np.float32)
+    tt = mlp_ref.two_term(a)
+    hi = mlp_ref.bf16_round(a)
+    assert np.all(np.abs(tt - a) <= 2.0 ** -16 * np.abs(a))         # about 16 significant bits
+    assert np.all(np.abs(hi - a) <= 2.0 ** -8 * np.abs(a))
+    assert np.abs(hi - a).max() > 100 * np.abs(tt - a).max()
+
+
+def test_two_term_forward_tracks_exact_math():
+    """With two-term inputs and activations the bf16 network's outputs stay within 1e-4 of the
+    exact network with the same (bf16) weights, where single-term bf16 activations do not."""
+    rng = np.random.default_rng(7)
+    params = mlp_ref.pack([(mlp_ref.bf16_round(W), b) for W, b in
+                           mlp_ref.unpack(rng.normal(0, 0.09, mlp_ref.param_count()))])
+    x = rng.uniform(-1, 1, (500, 9))
+    t = np.zeros((500, 9))
+    shift, scale = np.zeros(9), np.ones(9)
+    y_exact = mlp_ref.forward_backward(params, x, t, shift, scale)[2]
+    y_bf16 = mlp_ref.forward_backward(params, x, t, shift, scale, bf16=True)[2]
+    assert np.abs(y_bf16 - y_exact).max() < 1e-4
 
 
 def test_gauss_legendre_bin_rule():
