@@ -222,6 +222,117 @@ def run_mcmc(args, sync):
     return result
 
 
+def surrogate_comparison(args, sync):
+    """Binned probabilities of many hypotheses on the 2048^2 grid: the trained network with the
+    exact fallback (bins whose error-map bound exceeds 1e-3 / 1.5), against the exact
+    mixed-precision kernel alone (16 Gauss-Legendre nodes per bin, binned in registers) and the
+    plain FP64 kernel; max |dP| of the combined result against the exact binned values on
+    `surrogate_validate` hypotheses and against FP64 on one."""
+    import ctypes
+    import torch
+    from cudaprob3_amd import _lib
+    from cudaprob3_amd import surrogate as S
+    from cudaprob3_amd import train_surrogate as T
+    from cudaprob3_amd import workloads as W
+    from cudaprob3_amd.propagator import HipPropagatorSingle
+
+    n, n_hyp, slots = args.n, args.surrogate_hypotheses, 8
+    net, emap, meta = T.load(max_batch=8192)
+    cos = np.linspace(-1.0, 1.0, n)
+    centers = np.geomspace(0.05, 1000.0, n)
+    rng = np.random.Generator(np.random.PCG64(31))
+    hyps = [W.random_params(rng) for _ in range(n_hyp)]
+    nus = [int(rng.random() < 0.5) for _ in range(n_hyp)]
+    ev = S.GridEvaluator(net, emap, cos, centers, tol=1e-3, safety=1.5, slots=slots)
+    ex = S.ExactBinnedGrid(torch, cos, centers, slots=slots)
+    bins = n_hyp * n * n
+    ev.run(hyps[:slots], nus[:slots])                     # warm-up (code, graphs)
+    ex.run(hyps[:slots], nus[:slots])
+    sync()
+    t0 = time.perf_counter()
+    stats = ev.run(hyps, nus)
+    sync()
+    t_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ex.run(hyps, nus)
+    sync()
+    t_x = time.perf_counter() - t0
+    log(f"[bench] surrogate+fallback {bins / t_s:.4e} bins/s ({stats['exact_bins'] / bins:.3f} of bins exact), "
+        f"exact mixed binned {bins / t_x:.4e} bins/s")
+
+    # accuracy: combined result vs the exact binned kernel (fallback bins are identical)
+    n_val = min(args.surrogate_validate, n_hyp)
+    worst, sq, cnt = 0.0, 0.0, 0
+    for c0 in range(0, n_val, slots):
+        idx = list(range(c0, min(c0 + slots, n_val)))
+        keep = {}
+        ev.run([hyps[j] for j in idx], [nus[j] for j in idx],
+               consume=lambda i, out: keep.update(v=out.clone()))
+        def cmp(i, out):
+            nonlocal worst, sq, cnt
+            d = (keep["v"] - out).abs()
+            worst = max(worst, float(d.max()))
+            sq += float((d.double() ** 2).sum())
+            cnt += d.numel()
+        ex.run([hyps[j] for j in idx], [nus[j] for j in idx], consume=cmp)
+    log(f"[bench] surrogate+fallback vs exact binned on {n_val} hypotheses: max|dP| {worst:.3e}")
+
+    # one hypothesis in FP64 (hoisted kernel, unbinned, averaged here) and the plain FP64 rate
+    E_nodes, w = S.gl_energies(centers)
+    w64 = torch.as_tensor(w.astype(np.float64), device="cuda")
+    j = 0
+    radii, rhos = W.jitter12_model()
+    res = {}
+    for prec in ("fp64", "fp64_plain"):
+        p = HipPropagatorSingle(0, n, len(E_nodes), precision=prec)
+        p.setEnergyList(E_nodes)
+        p.setCosineList(cos)
+        p.setDensity(radii, rhos)
+        p.setProductionHeight(22.0)
+        p.setMNSMatrix(*hyps[j][:4])
+        p.setNeutrinoMasses(*hyps[j][4:])
+        p.calculateProbabilities(nus[j])
+        t0 = time.perf_counter()
+        p.calculateProbabilities(nus[j])
+        res[prec] = time.perf_counter() - t0
+        if prec == "fp64":
+            full = torch.empty((9, n, n, 16), dtype=torch.float64, device="cuda")
+            sync()
+            _lib.check(p._lib.cp3h_copy_results_device(p._handle(), ctypes.c_void_p(full.data_ptr()), full.numel()))
+            ref64 = full @ w64
+            del full
+        p.close()
+    keep = {}
+    ev.run([hyps[j]], [nus[j]], consume=lambda i, out: keep.update(s=out[0].double().clone()))
+    ex.run([hyps[j]], [nus[j]], consume=lambda i, out: keep.update(x=out[0].double().clone()))
+    d64_s = float((keep["s"] - ref64).abs().max())
+    d64_x = float((keep["x"] - ref64).abs().max())
+    del ref64, keep
+    log(f"[bench] hypothesis 0 vs FP64 binned: surrogate+fallback {d64_s:.3e}, exact mixed {d64_x:.3e}; "
+        f"fp64 {n * n / res['fp64']:.4e} bins/s, fp64_plain {n * n / res['fp64_plain']:.4e} bins/s")
+    classes = {k: ev.fallback_fraction(k) for k in range(S.N_CLASSES)}
+    ev.close()
+    ex.close()
+    net.close()
+    return {
+        "workload": f"{n_hyp} hypotheses (scan ranges, either ordering, nu/nubar) x {n} cosZ x {n} energy bins "
+                    "(+-2.5 %, 16-point Gauss-Legendre), jittered 12-shell Earth",
+        "bins_per_s": bins / t_s,
+        "exact_fraction": stats["exact_bins"] / bins,
+        "exact_fraction_by_class": classes,
+        "max_abs_dP_vs_exact_binned": worst, "rms_dP_vs_exact_binned": (sq / max(cnt, 1)) ** 0.5,
+        "validated_hypotheses": n_val,
+        "max_abs_dP_vs_fp64_hyp0": d64_s,
+        "exact_mixed_binned_bins_per_s": bins / t_x,
+        "exact_mixed_binned_max_abs_dP_vs_fp64_hyp0": d64_x,
+        "exact_fp64_bins_per_s": n * n / res["fp64"],
+        "exact_fp64_plain_bins_per_s": n * n / res["fp64_plain"],
+        "speedup_vs_exact_mixed": t_x / t_s,
+        "speedup_vs_exact_fp64_plain": (n * n / res["fp64_plain"]) ** -1 * bins / t_s,
+        "network": meta,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +347,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other precisions' throughput")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate-vs-exact comparison")
+    ap.add_argument("--surrogate-hypotheses", type=int, default=256)
+    ap.add_argument("--surrogate-validate", type=int, default=64,
+                    help="hypotheses whose every bin is checked against the exact binned kernel")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one elementwise copy of this many MiB (reads and writes that many "
                          "bytes), the known-size kernel that scales HBM counters read by a profiler")
@@ -335,6 +450,12 @@ def main():
             for q in qs:
                 q.close()
         result["other_precisions_pairs_per_s"] = extra
+
+    if not args.no_surrogate and args.precision == "mixed":
+        for p in props:
+            p.close()
+        props = []
+        result["surrogate"] = surrogate_comparison(args, sync)
 
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)

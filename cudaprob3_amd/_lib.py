@@ -49,6 +49,7 @@ SIGNATURES = {
     "cp3h_set_stream": (ctypes.c_int, [_handle, ctypes.c_void_p]),
     "cp3h_copy_results_device": (ctypes.c_int, [_handle, ctypes.c_void_p, ctypes.c_size_t]),
     "cp3h_sqdiff_results_f32": (ctypes.c_int, [_handle, ctypes.c_void_p, _c_double_p]),
+    "cp3h_compute_sqdiff_f32": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_void_p, _c_double_p]),
     "cp3h_last_compute_ms": (ctypes.c_int, [_handle, _c_float_p, _c_float_p]),
     "cp3h_last_error": (ctypes.c_char_p, []),
     "cp3h_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p,
@@ -71,6 +72,9 @@ SIGNATURES = {
     "cp3h_batch_device_results": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "cp3h_batch_bin_average_f32": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_float_p,
                                                   ctypes.c_void_p]),
+    "cp3h_batch_set_binning": (ctypes.c_int, [_handle, ctypes.c_int, _c_float_p, ctypes.c_void_p,
+                                              ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                                              _c_int_p]),
     "cp3h_mlp_param_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t)]),
     "cp3h_mlp_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_handle)]),
     "cp3h_mlp_destroy": (ctypes.c_int, [_handle]),
@@ -80,6 +84,8 @@ SIGNATURES = {
     "cp3h_mlp_get_params": (ctypes.c_int, [_handle, _c_float_p, ctypes.c_size_t]),
     "cp3h_mlp_forward": (ctypes.c_int, [_handle, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p]),
     "cp3h_mlp_set_forward_kernel": (ctypes.c_int, [_handle, ctypes.c_int]),
+    "cp3h_mlp_forward_grid": (ctypes.c_int, [_handle, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "cp3h_mlp_gradients": (ctypes.c_int, [_handle, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_float_p,
                                           ctypes.c_size_t, _c_float_p]),
     "cp3h_mlp_train_step": (ctypes.c_int, [_handle, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float,

@@ -139,6 +139,22 @@ class BatchPropagator:
         _lib.check(self._lib.cp3h_batch_set_params_many_f64(self._h, 0, len(self.sets), _dp(U), _dp(D),
                                                             nu.ctypes.data_as(ip), _dp(h)))
 
+    def set_binning(self, nodes: int, weights, out_ptr: int, offsets, plane_strides, row_strides):
+        """Store only bin averages: set i's (t, ic, bin) average goes to float element
+        offsets[i] + t * plane_strides[i] + ic * row_strides[i] + bin of the device buffer at
+        out_ptr.  nodes = 0 switches back to unbinned results."""
+        n = len(self.sets)
+        w = np.zeros(32, np.float32)
+        if nodes:
+            w[:nodes] = np.asarray(weights, np.float32)
+        off = np.ascontiguousarray(np.broadcast_to(np.asarray(offsets if nodes else 0, np.int64), n))
+        ps = np.ascontiguousarray(np.broadcast_to(np.asarray(plane_strides if nodes else 0, np.int64), n))
+        rs = np.ascontiguousarray(np.broadcast_to(np.asarray(row_strides if nodes else 0, np.int32), n))
+        llp = ctypes.POINTER(ctypes.c_longlong)
+        _lib.check(self._lib.cp3h_batch_set_binning(self._h, int(nodes), w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                    ctypes.c_void_p(out_ptr or None), off.ctypes.data_as(llp),
+                                                    ps.ctypes.data_as(llp), rs.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+
     def use_graph(self, enable: bool):
         _lib.check(self._lib.cp3h_batch_use_graph(self._h, int(bool(enable))))
 

@@ -1,6 +1,11 @@
 // Persistent pair kernel of a batched parameter scan: many parameter sets, each with its
 // own uneven grid and Earth model, propagated in one launch.
 //
+// In binned mode a set's energies are the quadrature nodes of its energy bins, bin-major, with
+// a power-of-two node count that divides 64: each aligned group of `nodes` lanes holds one
+// bin, reduces its weighted probabilities with butterfly lane exchanges, and its first lane
+// stores the nine bin averages (16x less output than the unbinned results).
+//
 // Work items are (set, 64-energy tile, cosine row), numbered set-major, then tile-major, so
 // consecutive items share a spectrum-table slice.  A fixed grid of workgroups (sized from
 // the CU count by the autotuner) walks the item list: wavefront w of the logical grid takes
@@ -25,7 +30,7 @@ __device__ inline int item_set(const long long* __restrict__ off, int n, long lo
     return lo;
 }
 
-template <int kWaves>
+template <int kWaves, bool kBinned>
 __global__ __launch_bounds__(kWaves * kWave) void batch_pairs(BatchDesc b) {
     const long long total = b.item_prefix[b.n_sets];
     const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
@@ -43,7 +48,22 @@ __global__ __launch_bounds__(kWaves * kWave) void batch_pairs(BatchDesc b) {
         const double* gl = b.geom + set.geom_off + std::size_t(ic) * (set.rows + 1);
         float p[3][3];
         propagate_pair_mixed(L, gl, b.Q + set.tab_off * 5, b.R + set.tab_off, set.ne_pad, ie, p);
-        if (ie < set.n_e) {
+        if (kBinned) {
+            // groups of `nodes` lanes are whole bins, all inside or all beyond n_e
+            const int nodes = b.bin_nodes;
+            const float w = b.bin_w[lane & (nodes - 1)];
+            float v[9];
+#pragma unroll
+            for (int t = 0; t < 9; t++) v[t] = w * p[t / 3][t % 3];
+            for (int off = nodes >> 1; off > 0; off >>= 1)
+#pragma unroll
+                for (int t = 0; t < 9; t++) v[t] += __shfl_xor(v[t], off);
+            if ((lane & (nodes - 1)) == 0 && ie < set.n_e) {
+                float* o = b.bin_out + set.bin_off + (long long)ic * set.bin_row + ie / nodes;
+#pragma unroll
+                for (int t = 0; t < 9; t++) o[t * set.bin_plane] = v[t];
+            }
+        } else if (ie < set.n_e) {
             const std::size_t plane = std::size_t(set.n_c) * set.n_e;
             float* o = b.out + set.out_off + std::size_t(ic) * set.n_e + ie;
 #pragma unroll
@@ -90,12 +110,18 @@ hipError_t launch_bin_average(const float* P, int n_c, int n_e, int nodes, const
 
 hipError_t launch_batch_pairs(const BatchDesc& b, int blocks, int waves, hipStream_t s) {
     if (blocks <= 0 || blocks % kXcds != 0) return hipErrorInvalidValue;
+    const int nodes = b.bin_nodes;
+    if (nodes < 0 || nodes > kMaxBinNodes || (nodes & (nodes - 1)) != 0) return hipErrorInvalidValue;
+#define CP3_BATCH(W)                                                                                     \
+    if (nodes > 0) hipLaunchKernelGGL((batch_pairs<W, true>), dim3(blocks), dim3(W * kWave), 0, s, b);  \
+    else hipLaunchKernelGGL((batch_pairs<W, false>), dim3(blocks), dim3(W * kWave), 0, s, b)
     switch (waves) {
-        case 4: hipLaunchKernelGGL(batch_pairs<4>, dim3(blocks), dim3(4 * kWave), 0, s, b); break;
-        case 8: hipLaunchKernelGGL(batch_pairs<8>, dim3(blocks), dim3(8 * kWave), 0, s, b); break;
-        case 16: hipLaunchKernelGGL(batch_pairs<16>, dim3(blocks), dim3(16 * kWave), 0, s, b); break;
+        case 4: CP3_BATCH(4); break;
+        case 8: CP3_BATCH(8); break;
+        case 16: CP3_BATCH(16); break;
         default: return hipErrorInvalidValue;
     }
+#undef CP3_BATCH
     return hipGetLastError();
 }
 

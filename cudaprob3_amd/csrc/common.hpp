@@ -83,6 +83,14 @@ hipError_t launch_pairs_f64(const GridDesc& g, int ne_pad, const double* geom, c
 
 // Number of per-workgroup partial sums launch_sqdiff_sum needs as scratch (plus one slot).
 int reduce_partials_needed();
+// sum of n device doubles in FP64 (fixed order); scratch: reduce_partials_needed() + 1
+// doubles, the sum lands in scratch[reduce_partials_needed()].
+hipError_t launch_sum_f64(const double* in, long long n, double* scratch, hipStream_t s);
+// Likelihood variant of the mixed pair kernel: per workgroup, the FP64 sum of (P - data)^2 over
+// its pairs and nine channels, into partials[likelihood_blocks(g)]; nothing else is stored.
+unsigned likelihood_blocks(const GridDesc& g);
+hipError_t launch_pairs_mixed_sqdiff(const GridDesc& g, int ne_pad, const double* geom, const float* Q,
+                                     const double* R, const float* data, double* partials, hipStream_t s);
 // sum_i (a_i - b_i)^2 in FP64 for device floats; result in partials[reduce_partials_needed()].
 hipError_t launch_sqdiff_sum(const float* a, const float* b, std::size_t n, double* partials, hipStream_t s);
 
@@ -99,8 +107,13 @@ struct BatchSet {
     long long geom_off;   // layer lengths, n_c * (rows + 1)
     long long tab_off;    // spectrum cells, (rows + 1) * ne_pad
     long long out_off;    // results, 9 * n_c * n_e floats (SoA per set)
+    // binned mode: bin average (t, ic, bin) -> bin_out[bin_off + t * bin_plane + ic * bin_row + bin]
+    long long bin_off, bin_plane;
+    int bin_row;
     int n_c, n_e, rows, ne_pad;
 };
+
+constexpr int kMaxBinNodes = 32;
 
 struct BatchDesc {
     const BatchSet* sets;
@@ -116,7 +129,10 @@ struct BatchDesc {
     double* geom;
     float4* Q;
     double2* R;
-    float* out;
+    float* out;                     // unbinned results (bin_nodes == 0)
+    float* bin_out;                 // binned mode: destination of the bin averages
+    int bin_nodes;                  // 0, or quadrature nodes per energy bin (a power of two <= 32)
+    float bin_w[kMaxBinNodes];      // quadrature weights of the nodes (summing to 1)
 };
 
 __device__ inline GridDesc batch_grid(const BatchDesc& b, const BatchSet& s) {
@@ -127,12 +143,13 @@ __device__ inline GridDesc batch_grid(const BatchDesc& b, const BatchSet& s) {
 // Set-up of every set (layer lengths + mixed spectrum tables), grid-stride over all cells.
 hipError_t launch_batch_setup(const BatchDesc& b, int blocks, hipStream_t s);
 // Persistent pair kernel over all sets: `blocks` workgroups of `waves` wavefronts
-// (waves in {4, 8, 16}); each wavefront takes (set, cosine row, 64-energy tile) items.
+// (waves in {4, 8, 16}); each wavefront takes (set, cosine row, 64-energy tile) items.  With
+// b.bin_nodes > 0 each set's energies are bin_nodes quadrature nodes per bin and only the
+// weighted bin averages are stored.
 hipError_t launch_batch_pairs(const BatchDesc& b, int blocks, int waves, hipStream_t s);
 
 // Energy-bin average of one set's SoA results whose n_e energies are `nodes` quadrature nodes
 // per bin: out[9][n_c][n_e / nodes].
-constexpr int kMaxBinNodes = 32;
 hipError_t launch_bin_average(const float* P, int n_c, int n_e, int nodes, const float* weights, float* out,
                               hipStream_t s);
 

@@ -27,6 +27,7 @@ struct cp3h_propagator {
     hipStream_t own_stream = nullptr;  // created and destroyed by the handle
     hipEvent_t ev_start = nullptr, ev_pairs = nullptr, ev_stop = nullptr;
     double* d_partials = nullptr;      // reduction scratch (cp3h_sqdiff_results_f32)
+    double* d_like = nullptr;          // per-workgroup sums of the likelihood kernel
     double* h_scalar = nullptr;        // pinned landing slot for a device scalar
     double* d_cos = nullptr;
     double* d_energies = nullptr;
@@ -139,6 +140,7 @@ int cp3h_destroy(cp3h_propagator* h) {
     free_dev(h->d_R);
     free_dev(h->d_out);
     free_dev(h->d_partials);
+    free_dev(h->d_like);
     if (h->h_scalar) (void)hipHostFree(h->h_scalar);
     if (h->ev_start) (void)hipEventDestroy(h->ev_start);
     if (h->ev_pairs) (void)hipEventDestroy(h->ev_pairs);
@@ -391,6 +393,48 @@ int cp3h_sqdiff_results_f32(cp3h_propagator* h, const float* data_device, double
     }
     CP3H_HIP(cp3::launch_sqdiff_sum(static_cast<const float*>(h->d_out), data_device, h->n_results(), h->d_partials,
                                     h->stream));
+    CP3H_HIP(hipMemcpyAsync(h->h_scalar, h->d_partials + cp3::reduce_partials_needed(), sizeof(double),
+                            hipMemcpyDeviceToHost, h->stream));
+    CP3H_HIP(hipStreamSynchronize(h->stream));
+    *result = *h->h_scalar;
+    return CP3H_OK;
+}
+
+int cp3h_compute_sqdiff_f32(cp3h_propagator* h, int antineutrino, double production_height_cm,
+                            const float* data_device, double* result) {
+    if (int rc = check_handle(h)) return rc;
+    if (h->precision != CP3H_MIXED) return fail(CP3H_ERR_STATE, "compute_sqdiff_f32 needs a CP3H_MIXED handle");
+    if (antineutrino != 0 && antineutrino != 1) return fail(CP3H_ERR_ARGUMENT, "neutrino type must be 0 or 1");
+    if (!h->has_energies || !h->has_cosines || !h->has_density || !h->has_params)
+        return fail(CP3H_ERR_STATE, "compute_sqdiff_f32: grid, density or oscillation parameters were not set");
+    if (!(production_height_cm >= 0.0)) return fail(CP3H_ERR_ARGUMENT, "production height must be >= 0");
+    if (!result || (h->n_results() > 0 && !data_device)) return fail(CP3H_ERR_ARGUMENT, "null pointer");
+    for (int i = 0; i < h->n_c; i++)
+        if (h->maxlayers[i] > h->rows)
+            return fail(CP3H_ERR_STATE, "maxlayers exceeds the number of shells of the density model");
+    CP3H_HIP(hipSetDevice(h->device));
+    h->params.antinu = antineutrino;
+    const cp3::GridDesc g{h->d_cos, h->d_energies, h->d_maxlayers, h->d_radii, h->d_rhos,
+                          h->n_c, h->n_e, h->rows, production_height_cm};
+    if (!h->d_partials) {
+        CP3H_HIP(hipMalloc(&h->d_partials, sizeof(double) * (cp3::reduce_partials_needed() + 1)));
+        CP3H_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->h_scalar), sizeof(double)));
+    }
+    const unsigned blocks = cp3::likelihood_blocks(g);
+    if (blocks == 0) {
+        *result = 0.0;
+        return CP3H_OK;
+    }
+    if (!h->d_like) CP3H_HIP(hipMalloc(&h->d_like, sizeof(double) * blocks));
+    CP3H_HIP(hipEventRecord(h->ev_start, h->stream));
+    CP3H_HIP(cp3::launch_setup_mixed(g, h->params, h->ne_pad, h->d_geom, static_cast<float*>(h->d_Q), h->d_R,
+                                     h->stream));
+    CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
+    CP3H_HIP(cp3::launch_pairs_mixed_sqdiff(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_R,
+                                            data_device, h->d_like, h->stream));
+    CP3H_HIP(hipEventRecord(h->ev_stop, h->stream));
+    h->has_timing = true;
+    CP3H_HIP(cp3::launch_sum_f64(h->d_like, blocks, h->d_partials, h->stream));
     CP3H_HIP(hipMemcpyAsync(h->h_scalar, h->d_partials + cp3::reduce_partials_needed(), sizeof(double),
                             hipMemcpyDeviceToHost, h->stream));
     CP3H_HIP(hipStreamSynchronize(h->stream));

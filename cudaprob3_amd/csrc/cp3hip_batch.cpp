@@ -1,7 +1,8 @@
 // Host side of the batched parameter-scan API (cp3h_batch_* in include/cp3hip.h).
 //
 // A batch owns concatenated device arrays for all sets (grids, Earth models, layer lengths,
-// mixed-precision spectrum tables, float results) sized once at creation, a pinned array
+// mixed-precision spectrum tables) sized once at creation, float results allocated at the
+// first unbinned computation (binned mode writes bin averages to the caller's buffer), a pinned array
 // of per-set descriptors (hypothesis + offsets) that is copied to the device at each
 // computation, one stream, and optionally an instantiated HIP graph of
 // {descriptor copy, set-up kernel, persistent pair kernel}.  A parameter-scan step writes
@@ -36,7 +37,11 @@ struct cp3h_batch {
     double* d_geom = nullptr;
     float4* d_Q = nullptr;
     double2* d_R = nullptr;
-    float* d_out = nullptr;
+    float* d_out = nullptr;             // unbinned results, allocated on first use
+    long long out_n = 0;
+    float* bin_out = nullptr;           // binned mode
+    int bin_nodes = 0;
+    float bin_w[cp3::kMaxBinNodes] = {};
     std::vector<char> has_grid, has_density, has_params;
     int waves = 8, groups_per_cu = 4;
     bool use_graph = false;
@@ -46,8 +51,10 @@ struct cp3h_batch {
 
     cp3::BatchDesc desc() const {
         const long long n1 = n_sets + 1;
-        return cp3::BatchDesc{d_sets, n_sets, d_prefix, d_prefix + n1, d_prefix + 2 * n1, d_cos, d_maxlayers,
-                              d_energies, d_radii, d_rhos, d_geom, d_Q, d_R, d_out};
+        cp3::BatchDesc d{d_sets, n_sets, d_prefix, d_prefix + n1, d_prefix + 2 * n1, d_cos, d_maxlayers,
+                         d_energies, d_radii, d_rhos, d_geom, d_Q, d_R, d_out, bin_out, bin_nodes, {}};
+        for (int g = 0; g < cp3::kMaxBinNodes; g++) d.bin_w[g] = bin_w[g];
+        return d;
     }
     int setup_blocks() const {
         const long long cells = geom_prefix.back() + spec_prefix.back();
@@ -75,6 +82,14 @@ template <typename T>
 void free_dev(T*& p) {
     if (p) (void)hipFree(p);
     p = nullptr;
+}
+
+// the unbinned result array, allocated the first time something needs it
+int ensure_out(cp3h_batch* b) {
+    if (b->d_out) return CP3H_OK;
+    CP3H_HIP(hipSetDevice(b->device));
+    CP3H_HIP(hipMalloc(&b->d_out, sizeof(float) * std::max(b->out_n, 1LL)));
+    return CP3H_OK;
 }
 
 void drop_graph(cp3h_batch* b) {
@@ -191,7 +206,7 @@ int cp3h_batch_create(int device, int n_sets, const int* n_cosines, const int* n
         CP3H_HIP(hipMalloc(&b->d_geom, sizeof(double) * std::max(geom_n, 1LL)));
         CP3H_HIP(hipMalloc(&b->d_Q, sizeof(float4) * 5 * std::max(tab_n, 1LL)));
         CP3H_HIP(hipMalloc(&b->d_R, sizeof(double2) * std::max(tab_n, 1LL)));
-        CP3H_HIP(hipMalloc(&b->d_out, sizeof(float) * std::max(out_n, 1LL)));
+        b->out_n = out_n;
         std::vector<long long> prefix;
         prefix.insert(prefix.end(), b->geom_prefix.begin(), b->geom_prefix.end());
         prefix.insert(prefix.end(), b->spec_prefix.begin(), b->spec_prefix.end());
@@ -321,6 +336,8 @@ int cp3h_batch_compute_async(cp3h_batch* b) {
         if (!b->has_grid[i] || !b->has_density[i] || !b->has_params[i])
             return fail(CP3H_ERR_STATE, "set " + std::to_string(i) + " is missing its grid, density or parameters");
     CP3H_HIP(hipSetDevice(b->device));
+    if (b->bin_nodes == 0)
+        if (int rc = ensure_out(b)) return rc;
     CP3H_HIP(hipEventRecord(b->ev_start, b->stream));
     if (b->use_graph) {
         if (!b->exec)
@@ -398,6 +415,7 @@ int cp3h_batch_copy_results_f32(cp3h_batch* b, int set, float* dst, size_t count
     const cp3::BatchSet& s = b->layout[set];
     if (count != std::size_t(9) * s.n_c * s.n_e) return fail(CP3H_ERR_ARGUMENT, "result count must be 9*n_c*n_e");
     if (count > 0 && !dst) return fail(CP3H_ERR_ARGUMENT, "null destination");
+    if (!b->d_out) return fail(CP3H_ERR_STATE, "no unbinned results have been computed");
     if (int rc = drain(b)) return rc;
     if (count) CP3H_HIP(hipMemcpy(dst, b->d_out + s.out_off, count * sizeof(float), hipMemcpyDeviceToHost));
     return CP3H_OK;
@@ -409,6 +427,7 @@ int cp3h_batch_bin_average_f32(cp3h_batch* b, int first, int count, int nodes, c
     if (first < 0 || count < 0 || first + count > b->n_sets) return fail(CP3H_ERR_ARGUMENT, "set range out of bounds");
     if (nodes < 1 || nodes > cp3::kMaxBinNodes || !weights || (count > 0 && !out_device))
         return fail(CP3H_ERR_ARGUMENT, "bad bin-average arguments");
+    if (count > 0 && !b->d_out) return fail(CP3H_ERR_STATE, "no unbinned results have been computed");
     CP3H_HIP(hipSetDevice(b->device));
     std::size_t off = 0;
     for (int i = first; i < first + count; i++) {
@@ -425,7 +444,44 @@ int cp3h_batch_bin_average_f32(cp3h_batch* b, int first, int count, int nodes, c
 int cp3h_batch_device_results(cp3h_batch* b, int set, const float** ptr) {
     if (int rc = check(b, set)) return rc;
     if (!ptr) return fail(CP3H_ERR_ARGUMENT, "null output pointer");
+    if (int rc = ensure_out(b)) return rc;
     *ptr = b->d_out + b->layout[set].out_off;
+    return CP3H_OK;
+}
+
+int cp3h_batch_set_binning(cp3h_batch* b, int nodes, const float* weights, float* out_device,
+                           const long long* offsets, const long long* plane_strides, const int* row_strides) {
+    if (!b) return fail(CP3H_ERR_ARGUMENT, "null batch handle");
+    if (nodes < 0 || nodes > cp3::kMaxBinNodes || (nodes & (nodes - 1)) != 0)
+        return fail(CP3H_ERR_ARGUMENT, "nodes per bin must be 0 or a power of two up to 32");
+    if (nodes > 0) {
+        if (!weights || !out_device || !offsets || !plane_strides || !row_strides)
+            return fail(CP3H_ERR_ARGUMENT, "null binning arrays");
+        for (int i = 0; i < b->n_sets; i++) {
+            const cp3::BatchSet& s = b->layout[i];
+            const long long bins = s.n_e / nodes;
+            if (s.n_e % nodes != 0)
+                return fail(CP3H_ERR_ARGUMENT, "set " + std::to_string(i) + ": energies are not a whole number of bins");
+            if (offsets[i] < 0 || row_strides[i] < bins ||
+                (s.n_c > 0 && plane_strides[i] < (long long)(s.n_c - 1) * row_strides[i] + bins))
+                return fail(CP3H_ERR_ARGUMENT, "set " + std::to_string(i) + ": bad output offset or strides");
+        }
+    }
+    if (int rc = drain(b)) return rc;
+    b->launched = false;
+    for (int i = 0; i < b->n_sets; i++) {
+        cp3::BatchSet& s = b->pinned[i];
+        s.bin_off = nodes > 0 ? offsets[i] : 0;
+        s.bin_plane = nodes > 0 ? plane_strides[i] : 0;
+        s.bin_row = nodes > 0 ? row_strides[i] : 0;
+    }
+    float w[cp3::kMaxBinNodes] = {};
+    for (int g = 0; g < nodes; g++) w[g] = weights[g];
+    // the kernel arguments change: a captured graph must be captured again
+    if (nodes != b->bin_nodes || out_device != b->bin_out || std::memcmp(w, b->bin_w, sizeof(w)) != 0) drop_graph(b);
+    b->bin_nodes = nodes;
+    b->bin_out = nodes > 0 ? out_device : nullptr;
+    std::memcpy(b->bin_w, w, sizeof(w));
     return CP3H_OK;
 }
 

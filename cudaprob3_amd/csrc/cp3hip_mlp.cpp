@@ -321,13 +321,28 @@ int cp3h_mlp_forward(cp3h_mlp* h, const float* x_device, long long n, float* y_d
     CP3H_HIP(hipSetDevice(h->device));
     if (h->forward_kernel != CP3H_MLP_LAYERED) {
         CP3H_HIP(cp3::launch_mlp_forward_fused(x_device, n, y_device, h->wb, h->b, h->scaling,
-                                               h->forward_kernel == CP3H_MLP_FUSED8 ? 2 : 1, h->stream));
+                                               h->forward_kernel == CP3H_MLP_FUSED8 ? 2 : 1, nullptr, h->stream));
         return CP3H_OK;
     }
     for (long long r0 = 0; r0 < n; r0 += h->max_batch) {
         const int M = int(std::min<long long>(h->max_batch, n - r0));
         if (int rc = forward(h, x_device + r0 * cp3::kMlpIn, M, nullptr, y_device + r0 * cp3::kMlpOut)) return rc;
     }
+    return CP3H_OK;
+}
+
+int cp3h_mlp_forward_grid(cp3h_mlp* h, const float* cos_device, int n_cos, const float* log10e_device, int n_bins,
+                          const float* hyp_device, int n_hyp, const int* skip_below_device, float* y_device) {
+    if (int rc = check(h)) return rc;
+    if (n_cos < 0 || n_bins < 0 || n_hyp < 0) return fail(CP3H_ERR_ARGUMENT, "negative grid size");
+    const long long n = (long long)n_cos * n_bins * n_hyp;
+    if (n == 0) return CP3H_OK;
+    if (!cos_device || !log10e_device || !hyp_device || !y_device)
+        return fail(CP3H_ERR_ARGUMENT, "null grid arrays");
+    CP3H_HIP(hipSetDevice(h->device));
+    const cp3::MlpGrid g{cos_device, log10e_device, hyp_device, skip_below_device, n_cos, n_bins};
+    CP3H_HIP(cp3::launch_mlp_forward_fused(nullptr, n, y_device, h->wb, h->b, h->scaling,
+                                           h->forward_kernel == CP3H_MLP_FUSED4 ? 1 : 2, &g, h->stream));
     return CP3H_OK;
 }
 

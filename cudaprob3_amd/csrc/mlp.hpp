@@ -68,9 +68,20 @@ hipError_t launch_sum_floats(const float* p, int n, float* out, hipStream_t s);
 // wb: bf16 weights [N][ld_wb], each row's K values at column 0 and again at column `dup`
 hipError_t launch_adam_layer(float* w, float* m1, float* m2, const float* g, int N, int K, int n_valid, int k_valid,
                              const AdamStep& a, __bf16* wb, int ld_wb, int dup, __bf16* wtb, hipStream_t s);
-// all layers of inference in one launch (mlp_infer.hip); row_groups 1 or 2 (4 or 8 wavefronts)
+// Grid-mode inference: rows (h, ic, ib) of n_hyp hypotheses on one cosZ x energy-bin grid.
+struct MlpGrid {
+    const float* cos;         // [n_cos]
+    const float* log10e;      // [n_bins] log10 of the bin centres / GeV
+    const float* hyp;         // [n_hyp][7]: theta12, theta13, theta23, deltaCP, dm21^2, dm32^2, nubar flag
+    const int* skip_below;    // optional [n_hyp][n_cos]: rows with ib below it may be left unwritten
+    int n_cos, n_bins;
+};
+
+// all layers of inference in one launch (mlp_infer.hip); row_groups 1 or 2 (4 or 8 wavefronts);
+// grid: null for row mode (x [n][9] -> y [n][9]), else grid mode (y [n_hyp][9][n_cos][n_bins])
 hipError_t launch_mlp_forward_fused(const float* x, long long n, float* y, const __bf16* const* w,
-                                    const float* const* b, const MlpScaling& sc, int row_groups, hipStream_t s);
+                                    const float* const* b, const MlpScaling& sc, int row_groups, const MlpGrid* grid,
+                                    hipStream_t s);
 hipError_t launch_adam_vec(float* w, float* m1, float* m2, const float* g, int n, int n_valid, const AdamStep& a,
                            hipStream_t s);
 

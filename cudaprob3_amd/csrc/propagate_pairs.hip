@@ -30,6 +30,7 @@
 // reused from that XCD's L2.  Layer lengths of the row are wave-uniform scalar loads.
 #include "common.hpp"
 #include "pair_mixed.hpp"
+#include "reduce.hpp"
 
 namespace cp3 {
 namespace {
@@ -177,31 +178,55 @@ struct MixedArgs {
     const float4* Q;          // [(rows + 1) * 5][ne_pad]
     const double2* R;         // [(rows + 1)][ne_pad]
     float* out;               // SoA [9][n_c][n_e]
+    const float* data;        // likelihood variant: SoA [9][n_c][n_e] reference values
+    double* partials;         // likelihood variant: one sum per workgroup
     int n_c, n_e, rows, ne_pad;
 };
 
+// kLike: instead of storing P, sum (P - data)^2 in FP64 over the workgroup's pairs (the
+// Markov-chain likelihood without a result round trip through HBM); every wavefront takes
+// part in the workgroup reduction, so none returns early.
+template <bool kLike>
 __global__ __launch_bounds__(kBlock) void pairs_mixed_kernel(MixedArgs a) {
     const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
     const int tiles = (a.n_e + kWave - 1) / kWave;
     const int row_groups = (a.n_c + 3) / 4;
     const int tile = lb / row_groups;
     const int ic = (lb % row_groups) * 4 + threadIdx.x / kWave;
-    if (tile >= tiles || ic >= a.n_c) return;
+    const bool valid = tile < tiles && ic < a.n_c;
+    if (!kLike && !valid) return;
     const int ie = tile * kWave + threadIdx.x % kWave;   // < ne_pad: table reads stay in bounds
-
-    const int L = a.maxlayers[ic];
-    const double* gl = a.geom + std::size_t(ic) * (a.rows + 1);
-    const std::size_t st = a.ne_pad;
-
-    float p[3][3];
-    propagate_pair_mixed(L, gl, a.Q, a.R, st, ie, p);
-    if (ie >= a.n_e) return;
     const std::size_t plane = std::size_t(a.n_c) * a.n_e;
-    float* o = a.out + std::size_t(ic) * a.n_e + ie;
+
+    double acc = 0.0;
+    if (valid) {
+        const int L = a.maxlayers[ic];
+        const double* gl = a.geom + std::size_t(ic) * (a.rows + 1);
+        const std::size_t st = a.ne_pad;
+        float p[3][3];
+        propagate_pair_mixed(L, gl, a.Q, a.R, st, ie, p);
+        if (ie < a.n_e) {
+            if (kLike) {
+                const float* d = a.data + std::size_t(ic) * a.n_e + ie;
 #pragma unroll
-    for (int in = 0; in < 3; in++)
+                for (int t = 0; t < 9; t++) {
+                    const double e = double(p[t / 3][t % 3]) - double(d[t * plane]);
+                    acc = __builtin_fma(e, e, acc);
+                }
+            } else {
+                float* o = a.out + std::size_t(ic) * a.n_e + ie;
 #pragma unroll
-        for (int ou = 0; ou < 3; ou++) o[(in * 3 + ou) * plane] = p[in][ou];
+                for (int in = 0; in < 3; in++)
+#pragma unroll
+                    for (int ou = 0; ou < 3; ou++) o[(in * 3 + ou) * plane] = p[in][ou];
+            }
+        }
+    }
+    if (kLike) {
+        __shared__ double lds[kBlock / kWave];
+        const double s = block_sum(acc, lds);
+        if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
+    }
 }
 
 hipError_t launch_pairs_fp64(const GridDesc& g, int ne_pad, const double* geom, const void* Q, const double* R,
@@ -216,14 +241,27 @@ hipError_t launch_pairs_fp64(const GridDesc& g, int ne_pad, const double* geom, 
 
 }  // namespace
 
+unsigned likelihood_blocks(const GridDesc& g) {
+    if (g.n_c == 0 || g.n_e == 0) return 0;
+    const unsigned tiles = (g.n_e + kWave - 1) / kWave;
+    return round_up(tiles * ((g.n_c + 3) / 4), kXcds);
+}
+
 hipError_t launch_pairs_mixed(const GridDesc& g, int ne_pad, const double* geom, const float* Q,
                               const double* R, float* out, hipStream_t s) {
     if (g.n_c == 0 || g.n_e == 0) return hipSuccess;
     MixedArgs a{g.maxlayers, geom, reinterpret_cast<const float4*>(Q), reinterpret_cast<const double2*>(R), out,
-                g.n_c, g.n_e, g.rows, ne_pad};
-    const unsigned tiles = (g.n_e + kWave - 1) / kWave;
-    const unsigned blocks = round_up(tiles * ((g.n_c + 3) / 4), kXcds);
-    hipLaunchKernelGGL(pairs_mixed_kernel, dim3(blocks), dim3(kBlock), 0, s, a);
+                nullptr, nullptr, g.n_c, g.n_e, g.rows, ne_pad};
+    hipLaunchKernelGGL(pairs_mixed_kernel<false>, dim3(likelihood_blocks(g)), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pairs_mixed_sqdiff(const GridDesc& g, int ne_pad, const double* geom, const float* Q,
+                                     const double* R, const float* data, double* partials, hipStream_t s) {
+    if (g.n_c == 0 || g.n_e == 0) return hipSuccess;
+    MixedArgs a{g.maxlayers, geom, reinterpret_cast<const float4*>(Q), reinterpret_cast<const double2*>(R), nullptr,
+                data, partials, g.n_c, g.n_e, g.rows, ne_pad};
+    hipLaunchKernelGGL(pairs_mixed_kernel<true>, dim3(likelihood_blocks(g)), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 

@@ -3,29 +3,12 @@
 // the host.  Two passes for a deterministic sum: per-workgroup partials, then one
 // workgroup adds the partials in a fixed order.
 #include "common.hpp"
+#include "reduce.hpp"
 
 namespace cp3 {
 namespace {
 
 constexpr int kReduceBlocks = 1024;
-
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    return v;
-}
-
-// Sum of v over the workgroup, valid in thread 0.
-__device__ inline double block_sum(double v, double* lds) {
-    v = wave_sum(v);
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    if (lane == 0) lds[wave] = v;
-    __syncthreads();
-    double s = 0.0;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < int(blockDim.x) / kWave; w++) s += lds[w];
-    return s;
-}
 
 __global__ __launch_bounds__(kBlock) void sqdiff_partials(const float* __restrict__ a, const float* __restrict__ b,
                                                           std::size_t n, double* __restrict__ partials) {
@@ -48,13 +31,15 @@ __global__ __launch_bounds__(kBlock) void sqdiff_partials(const float* __restric
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(kBlock) void sum_partials(const double* __restrict__ partials, int n,
+// out[blockIdx.x] = sum of in[i] over i = blockIdx.x * blockDim.x + threadIdx.x (mod the grid)
+__global__ __launch_bounds__(kBlock) void sum_partials(const double* __restrict__ in, long long n,
                                                        double* __restrict__ out) {
     __shared__ double lds[kBlock / kWave];
     double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partials[i];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) acc += in[i];
     const double s = block_sum(acc, lds);
-    if (threadIdx.x == 0) out[0] = s;
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
 }  // namespace
@@ -65,7 +50,17 @@ int reduce_partials_needed() { return kReduceBlocks; }
 // doubles of device scratch; the sum lands in partials[kReduceBlocks].
 hipError_t launch_sqdiff_sum(const float* a, const float* b, std::size_t n, double* partials, hipStream_t s) {
     hipLaunchKernelGGL(sqdiff_partials, dim3(kReduceBlocks), dim3(kBlock), 0, s, a, b, n, partials);
-    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kBlock), 0, s, partials, kReduceBlocks, partials + kReduceBlocks);
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kBlock), 0, s, partials, (long long)kReduceBlocks,
+                       partials + kReduceBlocks);
+    return hipGetLastError();
+}
+
+// sum of n doubles (any n >= 0) in two fixed-order passes; scratch: kReduceBlocks + 1 doubles,
+// the sum lands in scratch[kReduceBlocks].
+hipError_t launch_sum_f64(const double* in, long long n, double* scratch, hipStream_t s) {
+    hipLaunchKernelGGL(sum_partials, dim3(kReduceBlocks), dim3(kBlock), 0, s, in, n, scratch);
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kBlock), 0, s, scratch, (long long)kReduceBlocks,
+                       scratch + kReduceBlocks);
     return hipGetLastError();
 }
 

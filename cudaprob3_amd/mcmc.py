@@ -1,9 +1,10 @@
 """Markov-chain parameter fit over a fixed (cosZ, E) grid on one MI355X.
 
 Each step proposes new oscillation parameters (Gaussian, sigma = 1 % of each parameter's
-range), re-propagates the whole grid with the mixed-precision kernel, compares it with
-pseudo-data on the device (sum of squared differences in FP64, cp3h_sqdiff_results_f32) and
-accepts or rejects with the Metropolis rule.  Only one double per step crosses PCIe.
+range), re-propagates the whole grid with the mixed-precision kernel fused with the comparison
+against pseudo-data (sum of squared differences in FP64 per workgroup, cp3h_compute_sqdiff_f32:
+the probabilities are never written to HBM) and accepts or rejects with the Metropolis rule.
+Only one double per step crosses PCIe.
 """
 from __future__ import annotations
 
@@ -66,7 +67,14 @@ class Chain:
         p.calculateProbabilitiesAsync(self.grid.antinu)
 
     def loglike(self, params) -> float:
-        """-1/2 sum (P - D)^2 / sigma_P^2 over all nine channels and every pair."""
+        """-1/2 sum (P - D)^2 / sigma_P^2 over all nine channels and every pair (fused kernel)."""
+        p = self.prop
+        p.setMNSMatrix(*params[:4])
+        p.setNeutrinoMasses(*params[4:])
+        return -0.5 * p.computeSqdiff(self.grid.antinu, self.data.data_ptr()) / self.sigma_p ** 2
+
+    def loglike_stored(self, params) -> float:
+        """The same through stored results (calculateProbabilities, then the device comparison)."""
         self._evaluate(params)
         r = ctypes.c_double()
         _lib.check(self._lib.cp3h_sqdiff_results_f32(self.prop._handle(), ctypes.c_void_p(self.data.data_ptr()),

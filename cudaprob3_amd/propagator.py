@@ -290,6 +290,21 @@ class HipPropagatorSingle(Propagator):
             self._params_uploaded = True
         _lib.check(self._lib.cp3h_compute_async(h, int(type_), float(self.ProductionHeightinCentimeter)))
 
+    def computeSqdiff(self, type_, data_ptr: int) -> float:
+        """calculateProbabilities fused with sum (P - data)^2 over all results (FP64, on the
+        device; mixed precision only).  data_ptr: device float array in the results' SoA
+        layout.  The stored results are not updated."""
+        h = self._handle()
+        if not self.isSetProductionHeight:
+            raise RuntimeError("HipPropagatorSingle::calculateProbabilities. production height was not set")
+        if not getattr(self, "_params_uploaded", False):
+            self._upload_params(h)
+            self._params_uploaded = True
+        r = ctypes.c_double()
+        _lib.check(self._lib.cp3h_compute_sqdiff_f32(h, int(type_), float(self.ProductionHeightinCentimeter),
+                                                     ctypes.c_void_p(data_ptr), ctypes.byref(r)))
+        return r.value
+
     def _upload_params(self, h):
         if self.dtype == np.float64:
             U = np.ascontiguousarray(self.Mix_U, dtype=np.float64)

@@ -116,6 +116,19 @@ class SurrogateNet:
         self._before_torch()
         return out
 
+    def forward_grid(self, cos_t, log10e_t, hyp_t, skip_t, out_t):
+        """Grid-mode inference (asynchronous): out_t[h][t][ic][ib] for hypotheses hyp_t [n][7]
+        on cos_t x log10e_t; points with ib < skip_t[h][ic] are left to the exact kernel."""
+        n_hyp = hyp_t.shape[0]
+        assert out_t.numel() >= n_hyp * 9 * cos_t.numel() * log10e_t.numel()
+        assert skip_t is None or skip_t.shape[0] >= n_hyp
+        self._after_torch()
+        _lib.check(self._lib.cp3h_mlp_forward_grid(
+            self._h, ctypes.c_void_p(cos_t.data_ptr()), cos_t.numel(), ctypes.c_void_p(log10e_t.data_ptr()),
+            log10e_t.numel(), ctypes.c_void_p(hyp_t.data_ptr()), n_hyp,
+            ctypes.c_void_p(skip_t.data_ptr() if skip_t is not None else None), ctypes.c_void_p(out_t.data_ptr())))
+        self._before_torch()
+
     def sync(self):
         _lib.check(self._lib.cp3h_mlp_sync(self._h))
 
@@ -253,32 +266,181 @@ def train(net: SurrogateNet, data: BinnedSets, epochs: int, batch: int = 8192, l
 
 # --- error map and fallback ---------------------------------------------------------------------
 
+N_COS_CELLS = 32                 # error-map cells: 32 cosZ cells x 0.1-decade energy cells,
+E_CELL_DECADES = 0.1             # per hypothesis class (nu / nubar x normal / inverted ordering)
+N_E_CELLS = int(math.ceil((LOG10E_RANGE[1] - LOG10E_RANGE[0]) / E_CELL_DECADES - 1e-9))
+N_CLASSES = 4
+
+
+def hypothesis_class(params, antinu) -> int:
+    return 2 * int(antinu) + int(params[5] < 0)
+
+
 @dataclass
 class ErrorMap:
-    n_cos: int
-    n_e: int
-    max_err: np.ndarray      # [n_cos][n_e] largest |dP| seen per cell (all nine channels)
+    max_err: np.ndarray      # [class][cos cell][energy cell]: largest held-out |dP| (any channel)
 
-    def cell(self, cos, log10E):
-        ic = np.clip(((np.asarray(cos) + 1.0) / 2.0 * self.n_cos).astype(int), 0, self.n_cos - 1)
-        ie = np.clip(((np.asarray(log10E) - LOG10E_RANGE[0]) / (LOG10E_RANGE[1] - LOG10E_RANGE[0]) * self.n_e)
-                     .astype(int), 0, self.n_e - 1)
+    @staticmethod
+    def cells(cos, log10E):
+        ic = np.clip(((np.asarray(cos) + 1.0) / 2.0 * N_COS_CELLS).astype(int), 0, N_COS_CELLS - 1)
+        ie = np.clip(((np.asarray(log10E) - LOG10E_RANGE[0]) / E_CELL_DECADES).astype(int), 0, N_E_CELLS - 1)
         return ic, ie
 
-    def fallback_mask(self, tol: float, safety: float = 2.0) -> np.ndarray:
-        return self.max_err * safety > tol
+    def fallback(self, k: int, cos, log10E, tol: float, safety: float) -> np.ndarray:
+        """[n_cos][n_bins] mask of the grid points whose cell bound exceeds the tolerance."""
+        ic, _ = self.cells(cos, np.zeros(1))
+        _, ie = self.cells(np.zeros(1), log10E)
+        return self.max_err[k][ic[:, None], ie[None, :]] * safety > tol
 
 
-def build_error_map(net: SurrogateNet, data: BinnedSets, n_cos=32, n_e=32) -> ErrorMap:
+def build_error_map(net: SurrogateNet, data: BinnedSets) -> ErrorMap:
+    """Largest |dP| of the network on held-out exact data, per class and cell."""
     torch = net.torch
     Y = net.forward(data.X)
     net.sync()
     err = (Y - data.T).abs().amax(dim=1)
-    cos = data.X[:, 0]
-    l10 = data.X[:, 1]
-    ic = ((cos + 1.0) / 2.0 * n_cos).long().clamp(0, n_cos - 1)
-    ie = ((l10 - LOG10E_RANGE[0]) / (LOG10E_RANGE[1] - LOG10E_RANGE[0]) * n_e).long().clamp(0, n_e - 1)
-    cell = ic * n_e + ie
-    m = torch.zeros(n_cos * n_e, dtype=torch.float32, device=err.device)
+    X = data.X
+    ic = ((X[:, 0] + 1.0) / 2.0 * N_COS_CELLS).long().clamp(0, N_COS_CELLS - 1)
+    ie = ((X[:, 1] - LOG10E_RANGE[0]) / E_CELL_DECADES).long().clamp(0, N_E_CELLS - 1)
+    k = 2 * X[:, 8].round().long() + (X[:, 7] < 0).long()
+    cell = (k * N_COS_CELLS + ic) * N_E_CELLS + ie
+    m = torch.zeros(N_CLASSES * N_COS_CELLS * N_E_CELLS, dtype=torch.float32, device=err.device)
     m.scatter_reduce_(0, cell, err, reduce="amax")
-    return ErrorMap(n_cos, n_e, m.view(n_cos, n_e).cpu().numpy())
+    return ErrorMap(m.view(N_CLASSES, N_COS_CELLS, N_E_CELLS).cpu().numpy())
+
+
+# --- evaluation on a fixed grid ------------------------------------------------------------------
+
+def _hyp_row(params, antinu):
+    return [float(x) for x in params] + [float(antinu)]
+
+
+class GridEvaluator:
+    """Bin-averaged probabilities of many hypotheses on one cosZ x energy-bin grid (the jittered
+    12-shell Earth): the network fills the grid (fused kernel, grid mode) except, per cosine row,
+    the leading run of energy bins whose error-map cells exceed tol / safety; the exact
+    mixed-precision kernel computes those bins (batched, binned mode: 16 Gauss-Legendre nodes
+    per bin, averaged in registers) into the same array at the same time, on its own stream.
+
+    Hypotheses are processed `slots` at a time per class; results go to ``self.out``
+    [slots][9][n_cos][n_bins] (t = in*3 + out) and are handed to ``consume``.
+    """
+
+    def __init__(self, net: SurrogateNet, emap: ErrorMap, cos, centers, tol: float = 1e-3, safety: float = 1.5,
+                 slots: int = 8, device: int = 0):
+        torch = net.torch
+        self.torch, self.net, self.slots = torch, net, slots
+        self.cos = np.asarray(cos, np.float64)
+        self.centers = np.asarray(centers, np.float64)
+        n_c, n_b = len(self.cos), len(self.centers)
+        self.n_c, self.n_b, self.plane = n_c, n_b, n_c * n_b
+        dev = torch.device(f"cuda:{device}")
+        self.dev = dev
+        l10 = np.log10(self.centers)
+        self.cos_t = torch.as_tensor(self.cos.astype(np.float32), device=dev)
+        self.l10_t = torch.as_tensor(l10.astype(np.float32), device=dev)
+        self.out = torch.empty((slots, 9, n_c, n_b), dtype=torch.float32, device=dev)
+        radii, rhos = W.jitter12_model()
+        E_nodes, self.weights = gl_energies(self.centers)
+        self.skip, self.skip_t, self.batches, self.runs = [], [], [], []
+        for k in range(N_CLASSES):
+            mask = emap.fallback(k, self.cos, l10, tol, safety)
+            last = np.where(mask.any(axis=1), n_b - np.argmax(mask[:, ::-1], axis=1), 0)   # prefix lengths
+            self.skip.append(last.astype(np.int32))
+            self.skip_t.append(torch.as_tensor(np.tile(last.astype(np.int32), (slots, 1)), device=dev))
+            runs = []          # consecutive cosine rows with the same prefix -> one exact set
+            c0 = 0
+            for c in range(1, n_c + 1):
+                if c == n_c or last[c] != last[c0]:
+                    if last[c0] > 0:
+                        runs.append((c0, c, int(last[c0])))
+                    c0 = c
+            self.runs.append(runs)
+            if not runs:
+                self.batches.append(None)
+                continue
+            dummy = (0.59, 0.15, 0.78, 0.0, 7.4e-5, 2.5e-3 if k % 2 == 0 else -2.5e-3)
+            specs = [SetSpec(self.cos[a:b], E_nodes[:16 * kb], radii, rhos, dummy, k // 2)
+                     for _ in range(slots) for (a, b, kb) in runs]
+            bp = BatchPropagator(specs, device=device, graph=True)
+            offs = [s * 9 * self.plane + a * n_b for s in range(slots) for (a, _, _) in runs]
+            bp.set_binning(16, self.weights, self.out.data_ptr(), offs, self.plane, n_b)
+            self.batches.append(bp)
+        torch.cuda.synchronize(dev)
+
+    def fallback_fraction(self, k: int) -> float:
+        return float(self.skip[k].sum()) / self.plane
+
+    def run(self, params, antinu, consume=None) -> dict:
+        """Evaluates every hypothesis; returns counts (bins, exact bins) per class."""
+        torch, net = self.torch, self.net
+        order = sorted(range(len(params)), key=lambda i: hypothesis_class(params[i], antinu[i]))
+        stats = {"bins": 0, "exact_bins": 0}
+        i = 0
+        while i < len(order):
+            k = hypothesis_class(params[order[i]], antinu[order[i]])
+            chunk = []
+            while i < len(order) and len(chunk) < self.slots and \
+                    hypothesis_class(params[order[i]], antinu[order[i]]) == k:
+                chunk.append(order[i])
+                i += 1
+            rows = [_hyp_row(params[j], antinu[j]) for j in chunk]
+            rows += [rows[-1]] * (self.slots - len(chunk))        # idle slots repeat the last one
+            hyp = torch.tensor(rows, dtype=torch.float32, device=self.dev)
+            torch.cuda.synchronize(self.dev)          # previous chunk consumed; hyp uploaded
+            net.forward_grid(self.cos_t, self.l10_t, hyp, self.skip_t[k], self.out)
+            bp = self.batches[k]
+            if bp is not None:
+                runs = self.runs[k]
+                p = np.repeat(np.array([r[:6] for r in rows], np.float64), len(runs), axis=0)
+                bp.set_params_many(p, np.full(len(p), k // 2, np.int32))
+                bp.compute_async()
+            net.sync()
+            if bp is not None:
+                bp.wait()
+            stats["bins"] += len(chunk) * self.plane
+            stats["exact_bins"] += len(chunk) * int(self.skip[k].sum())
+            if consume is not None:
+                consume(chunk, self.out[:len(chunk)])
+        return stats
+
+    def close(self):
+        for bp in self.batches:
+            if bp is not None:
+                bp.close()
+        self.batches = []
+
+
+class ExactBinnedGrid:
+    """The exact mixed-precision kernel alone on the same grid: every bin from 16 GL nodes
+    (batched, binned mode), `slots` hypotheses per launch, into ``self.out``."""
+
+    def __init__(self, torch, cos, centers, slots: int = 8, device: int = 0):
+        self.torch, self.slots = torch, slots
+        self.cos = np.asarray(cos, np.float64)
+        self.centers = np.asarray(centers, np.float64)
+        n_c, n_b = len(self.cos), len(self.centers)
+        self.plane = n_c * n_b
+        self.dev = torch.device(f"cuda:{device}")
+        self.out = torch.empty((slots, 9, n_c, n_b), dtype=torch.float32, device=self.dev)
+        radii, rhos = W.jitter12_model()
+        E_nodes, w = gl_energies(self.centers)
+        specs = [SetSpec(self.cos, E_nodes, radii, rhos, (0.59, 0.15, 0.78, 0.0, 7.4e-5, 2.5e-3), 0)
+                 for _ in range(slots)]
+        self.bp = BatchPropagator(specs, device=device, graph=True)
+        self.bp.set_binning(16, w, self.out.data_ptr(), [s * 9 * self.plane for s in range(slots)], self.plane, n_b)
+        torch.cuda.synchronize(self.dev)
+
+    def run(self, params, antinu, consume=None):
+        for c0 in range(0, len(params), self.slots):
+            idx = list(range(c0, min(c0 + self.slots, len(params))))
+            sel = idx + [idx[-1]] * (self.slots - len(idx))
+            self.torch.cuda.synchronize(self.dev)
+            self.bp.set_params_many(np.array([params[j] for j in sel], np.float64),
+                                    np.array([antinu[j] for j in sel], np.int32))
+            self.bp.compute()
+            if consume is not None:
+                consume(idx, self.out[:len(idx)])
+
+    def close(self):
+        self.bp.close()

@@ -1,0 +1,74 @@
+"""Trains the surrogate network on exact bin-averaged probabilities and writes its parameters
+and error map (cudaprob3_amd/data/surrogate_jitter12.npz by default).
+
+    python -m cudaprob3_amd.train_surrogate [--samples 50331648] [--epochs 40] [--min-energy 2]
+
+Training data: `samples` points (cosZ, energy-bin centre, hypothesis) in sets of 64 random
+cosines x 64 log-uniform bin centres per random hypothesis (parameter ranges of the scan
+workload, either ordering, nu or nubar), targets from the exact mixed-precision kernel
+averaged over +-2.5 % bins by 16-point Gauss-Legendre quadrature, on the jittered 12-shell
+Earth.  Bins below --min-energy GeV are left out of training (they always fall back to the
+exact kernel).  The error map comes from 1024 further hypotheses over the full energy range.
+Runs on one GPU in a few minutes.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import surrogate as S
+
+DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_jitter12.npz")
+
+
+def save(path, net, emap: S.ErrorMap, meta: dict):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.savez_compressed(path, params=net.get_params(), max_err=emap.max_err.astype(np.float32),
+                        feature_lo=S.FEATURE_LO, feature_hi=S.FEATURE_HI, meta=np.array(json.dumps(meta)))
+
+
+def load(path: str = DEFAULT_PATH, max_batch: int = 8192, device: int = 0):
+    """(network for inference, error map, metadata) from a file written by save()."""
+    with np.load(path, allow_pickle=False) as z:
+        if not (np.array_equal(z["feature_lo"], S.FEATURE_LO) and np.array_equal(z["feature_hi"], S.FEATURE_HI)):
+            raise ValueError(f"{path}: input scaling differs from this build's")
+        net = S.SurrogateNet(max_batch=max_batch, training=False, device=device)
+        net.set_params(z["params"])
+        return net, S.ErrorMap(z["max_err"].astype(np.float64)), json.loads(str(z["meta"]))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--samples", type=int, default=50_331_648)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--min-energy", type=float, default=2.0)
+    ap.add_argument("--seed", type=int, default=101)
+    ap.add_argument("--out", default=DEFAULT_PATH)
+    a = ap.parse_args(argv)
+    t0 = time.time()
+    lo = math.log10(max(a.min_energy, 10 ** S.LOG10E_RANGE[0]))
+    n_sets = a.samples // 4096
+    data = S.make_dataset(n_sets=n_sets, n_cos=64, n_bins=64, seed=a.seed, log10e_range=(lo, S.LOG10E_RANGE[1]))
+    print(f"training data: {data.X.shape[0]} samples from {n_sets} hypotheses ({time.time() - t0:.1f} s)", flush=True)
+    net = S.SurrogateNet(max_batch=65536, training=True, seed=3)
+    t1 = time.time()
+    hist = S.train(net, data, epochs=a.epochs, batch=a.batch, lr_max=a.lr, seed=a.seed,
+                   log=lambda s: print(f"[{time.time() - t1:.0f} s] {s}", flush=True))
+    del data
+    held = S.make_dataset(n_sets=1024, n_cos=64, n_bins=64, seed=a.seed + 1)
+    emap = S.build_error_map(net, held)
+    meta = dict(samples=n_sets * 4096, epochs=a.epochs, batch=a.batch, lr=a.lr, min_energy_gev=10 ** lo, seed=a.seed,
+                final_loss=hist[-1] if hist else None, train_seconds=time.time() - t1, held_out_samples=held.X.shape[0])
+    save(a.out, net, emap, meta)
+    print(json.dumps(meta), flush=True)
+
+
+if __name__ == "__main__":
+    main()

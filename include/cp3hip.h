@@ -114,6 +114,14 @@ int cp3h_copy_results_device(cp3h_propagator* h, void* dst_device, size_t count)
    after the last compute on the handle's stream and waits for the scalar. */
 int cp3h_sqdiff_results_f32(cp3h_propagator* h, const float* data_device, double* result);
 
+/* calculateProbabilities fused with that comparison (no counterpart in the reference): the FP64
+   sum over all 9*n_c*n_e probabilities of (P - data)^2 for the current parameters, computed
+   by the mixed kernel without storing P (the handle's results are left as they were);
+   CP3H_MIXED handles; waits for the result.  The pair-kernel time is reported by
+   cp3h_last_compute_ms. */
+int cp3h_compute_sqdiff_f32(cp3h_propagator* h, int antineutrino, double production_height_cm,
+                            const float* data_device, double* result);
+
 /* GPU time of the last compute, from HIP events on the handle's stream (waits for it):
    total_ms covers all of its kernels, pairs_ms only the per-pair propagation kernel
    (for CP3H_FP64_PLAIN the two are the same kernel).  Either pointer may be NULL. */
@@ -179,6 +187,15 @@ int cp3h_batch_device_results(cp3h_batch* b, int set, const float** ptr);
 int cp3h_batch_bin_average_f32(cp3h_batch* b, int first, int count, int nodes, const float* weights,
                                float* out_device);
 
+/* Binned mode (no counterpart in the reference): with nodes > 0 every set's energy list must
+   be `nodes` quadrature nodes per bin (bin-major, nodes a power of two <= 32) and each
+   computation stores only the bin averages sum_g weights[g] * P(node g), for set i at
+   out_device[offsets[i] + t * plane_strides[i] + ic * row_strides[i] + bin] (t = in*3+out), so
+   many sets can fill disjoint blocks of one larger [9][rows][bins] array.  No unbinned results
+   are kept.  nodes = 0 returns to unbinned results. */
+int cp3h_batch_set_binning(cp3h_batch* b, int nodes, const float* weights, float* out_device,
+                           const long long* offsets, const long long* plane_strides, const int* row_strides);
+
 /* ---- Surrogate network --------------------------------------------------------------------
  * No counterpart in the reference.  A 9 -> 256 x 5 (GELU, tanh form) -> 9 MLP with BF16
  * weights and FP32 accumulation on the matrix cores, approximating energy-bin-averaged
@@ -207,6 +224,14 @@ int cp3h_mlp_forward(cp3h_mlp* h, const float* x_device, long long n, float* y_d
    training path's per-layer GEMMs in chunks of max_batch rows. */
 enum { CP3H_MLP_LAYERED = 0, CP3H_MLP_FUSED4 = 1, CP3H_MLP_FUSED8 = 2 };
 int cp3h_mlp_set_forward_kernel(cp3h_mlp* h, int kernel);
+/* Inference on n_hyp hypotheses over one cosZ x energy-bin grid (fused kernel): the inputs
+   of point (h, ic, ib) are cos[ic], log10e[ib] (log10 of the bin centre / GeV) and hyp[h][0..6]
+   (theta12, theta13, theta23, deltaCP, dm21^2, dm32^2, nubar flag); outputs go to
+   y[h][t][ic][ib], the propagators' SoA layout.  skip_below (device [n_hyp][n_cos], may be
+   NULL): points with ib < skip_below[h][ic] are left unwritten (bins another kernel fills,
+   possibly at the same time). */
+int cp3h_mlp_forward_grid(cp3h_mlp* h, const float* cos_device, int n_cos, const float* log10e_device, int n_bins,
+                          const float* hyp_device, int n_hyp, const int* skip_below_device, float* y_device);
 /* gradients of the mean squared error over the 9 outputs of one batch (M a multiple of 256,
    <= max_batch), copied to host memory in the parameter layout, and the loss */
 int cp3h_mlp_gradients(cp3h_mlp* h, const float* x_device, const float* t_device, int M, float* grads, size_t n,

@@ -104,3 +104,66 @@ def test_scan1024_full_size(oracle_mod):
             rows = np.arange(0, len(s.cosines), 23)
             ref = oracle_mod.propagate(s.antinu, s.cosines[rows], s.energies, s.radii, s.rhos, s.h_km, s.params)
             assert np.abs(got[:, rows, :].transpose(1, 2, 0) - ref).max() <= MIXED_TOL
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_binned_mode_matches_unbinned_average(oracle_mod, graph):
+    """Binned mode (bin averages reduced across lanes in the pair kernel, written into disjoint
+    blocks of one strided array) against the unbinned results averaged afterwards and against
+    the oracle's FP64 average; cells outside the blocks stay untouched."""
+    import torch
+    from cudaprob3_amd.surrogate import gl_energies
+    rng = np.random.Generator(np.random.PCG64(8))
+    radii, rhos = W.jitter12_model()
+    sets, shapes = [], []
+    for i in range(6):
+        n_c, n_b = int(rng.integers(1, 9)), int(rng.integers(1, 11))
+        cos = np.sort(rng.uniform(-1, 1, n_c))
+        centers = np.sort(10 ** rng.uniform(-1.3, 3, n_b))
+        E, w = gl_energies(centers)
+        sets.append(SetSpec(cos, E, radii, rhos, W.random_params(rng), antinu=i % 2))
+        shapes.append((n_c, n_b))
+    print(f"[gpu] binned batch mode, graph={graph}", flush=True)
+    # set i fills rows [r0_i, r0_i + n_c) x bins [0, n_b) of a [9][R][B] array with B = 16
+    R, B = sum(s[0] for s in shapes) + 3, 16
+    out = torch.full((9, R, B), -7.0, dtype=torch.float32, device="cuda")
+    r0 = np.cumsum([0] + [s[0] for s in shapes])[:-1] + 1
+    with BatchPropagator(sets, graph=False) as b:
+        b.compute()
+        plain = [b.results(i) for i in range(len(sets))]
+    with BatchPropagator(sets, graph=graph) as b:
+        torch.cuda.synchronize()
+        b.set_binning(16, w, out.data_ptr(), r0 * B, R * B, B)
+        b.compute()
+        b.compute()                         # a replay writes the same values
+        got = out.cpu().numpy()
+    mask = np.zeros((R, B), bool)
+    for i, (s, (n_c, n_b)) in enumerate(zip(sets, shapes)):
+        blk = got[:, r0[i]:r0[i] + n_c, :n_b]
+        avg = np.einsum("tcbg,g->tcb", plain[i].reshape(9, n_c, n_b, 16).astype(np.float64), w.astype(np.float64))
+        assert np.abs(blk - avg).max() <= 2e-6
+        ref = oracle_mod.propagate(s.antinu, s.cosines, s.energies, radii, rhos, 22.0, s.params)
+        ref = np.einsum("cbgt,g->tcb", ref.reshape(n_c, n_b, 16, 9), w.astype(np.float64))
+        assert np.abs(blk - ref).max() <= MIXED_TOL
+        mask[r0[i]:r0[i] + n_c, :n_b] = True
+    assert np.all(got[:, ~mask] == -7.0)
+
+
+def test_binned_mode_rejects_bad_layouts():
+    from cudaprob3_amd._lib import Cp3hError
+    from cudaprob3_amd.surrogate import gl_energies
+    radii, rhos = W.jitter12_model()
+    E, w = gl_energies([1.0, 2.0])
+    with BatchPropagator([SetSpec(np.array([-0.5, 0.5]), E, radii, rhos, W.NUFIT)], graph=False) as b:
+        import torch
+        out = torch.zeros(64, device="cuda")
+        with pytest.raises(Cp3hError):
+            b.set_binning(3, w[:3], out.data_ptr(), [0], 4, 2)          # not a power of two
+        b.set_binning(32, np.ones(32) / 32, out.data_ptr(), [0], 2, 1)   # 32 energies = one bin per row
+        with pytest.raises(Cp3hError):
+            b.set_binning(16, w, out.data_ptr(), [0], 4, 1)            # row stride below the bin count
+        with pytest.raises(Cp3hError):
+            b.set_binning(16, w, out.data_ptr(), [0], 2, 2)            # planes overlap
+        b.set_binning(0, None, 0, 0, 0, 0)                              # back to unbinned results
+        b.compute()
+        assert b.results(0).shape == (9, 2, 32)
