@@ -240,9 +240,32 @@ def surrogate_comparison(args, sync):
     net, emap, meta = T.load(max_batch=8192)
     cos = np.linspace(-1.0, 1.0, n)
     centers = np.geomspace(0.05, 1000.0, n)
-    rng = np.random.Generator(np.random.PCG64(31))
+    rng = np.random.Generator(np.random.PCG64(44))
     hyps = [W.random_params(rng) for _ in range(n_hyp)]
     nus = [int(rng.random() < 0.5) for _ in range(n_hyp)]
+    # the network kernel alone (grid mode, no skipped bins), timed with HIP events on its stream
+    from cudaprob3_amd import metrics as M
+    dev = torch.device("cuda:0")
+    cos_t = torch.as_tensor(cos.astype(np.float32), device=dev)
+    l10_t = torch.as_tensor(np.log10(centers).astype(np.float32), device=dev)
+    hyp_t = torch.tensor([list(hyps[j]) + [nus[j]] for j in range(slots)], dtype=torch.float32, device=dev)
+    y_t = torch.empty((slots, 9, n, n), dtype=torch.float32, device=dev)
+    net.forward_grid(cos_t, l10_t, hyp_t, None, y_t)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 3
+    sync()
+    e0.record(net.stream)
+    for _ in range(reps):
+        net.forward_grid(cos_t, l10_t, hyp_t, None, y_t)
+    e1.record(net.stream)
+    sync()
+    mlp_ms = e0.elapsed_time(e1) / reps
+    rows = slots * n * n
+    mlp_tflops = rows * M.MLP_FLOPS_PER_ROW / (mlp_ms * 1e-3) / 1e12
+    del y_t
+    log(f"[bench] network kernel: {rows / (mlp_ms * 1e-3):.4e} rows/s, {mlp_tflops:.0f} TFLOP/s "
+        f"({mlp_tflops / M.PEAK_BF16_MFMA_TFLOPS:.3f} of the BF16 MFMA peak)")
+
     ev = S.GridEvaluator(net, emap, cos, centers, tol=1e-3, safety=1.5, slots=slots)
     ex = S.ExactBinnedGrid(torch, cos, centers, slots=slots)
     bins = n_hyp * n * n
@@ -266,14 +289,17 @@ def surrogate_comparison(args, sync):
     for c0 in range(0, n_val, slots):
         idx = list(range(c0, min(c0 + slots, n_val)))
         keep = {}
+        # the evaluator hands results over per class, in its own order: key them by hypothesis
         ev.run([hyps[j] for j in idx], [nus[j] for j in idx],
-               consume=lambda i, out: keep.update(v=out.clone()))
-        def cmp(i, out):
+               consume=lambda sub, out: keep.update({sub[m]: out[m].clone() for m in range(len(sub))}))
+
+        def cmp(sub, out):
             nonlocal worst, sq, cnt
-            d = (keep["v"] - out).abs()
-            worst = max(worst, float(d.max()))
-            sq += float((d.double() ** 2).sum())
-            cnt += d.numel()
+            for m, jj in enumerate(sub):
+                d = (keep.pop(jj) - out[m]).abs()
+                worst = max(worst, float(d.max()))
+                sq += float((d.double() ** 2).sum())
+                cnt += d.numel()
         ex.run([hyps[j] for j in idx], [nus[j] for j in idx], consume=cmp)
     log(f"[bench] surrogate+fallback vs exact binned on {n_val} hypotheses: max|dP| {worst:.3e}")
 
@@ -299,7 +325,8 @@ def surrogate_comparison(args, sync):
             full = torch.empty((9, n, n, 16), dtype=torch.float64, device="cuda")
             sync()
             _lib.check(p._lib.cp3h_copy_results_device(p._handle(), ctypes.c_void_p(full.data_ptr()), full.numel()))
-            ref64 = full @ w64
+            ref64 = (full * w64).sum(dim=-1)        # elementwise: torch's batched matmul over this
+            #                                           4.8 GB float64 operand returned wrong values
             del full
         p.close()
     keep = {}
@@ -330,6 +357,11 @@ def surrogate_comparison(args, sync):
         "speedup_vs_exact_mixed": t_x / t_s,
         "speedup_vs_exact_fp64_plain": (n * n / res["fp64_plain"]) ** -1 * bins / t_s,
         "network": meta,
+        "network_roofline": {"bound": "mfma", "kernel": "mlp_forward_fused (grid mode)",
+                             "achieved": mlp_tflops, "peak": M.PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
+                             "frac": mlp_tflops / M.PEAK_BF16_MFMA_TFLOPS, "kernel_ms": mlp_ms,
+                             "rows_per_launch": rows, "algorithmic_flops_per_row": M.MLP_FLOPS_PER_ROW,
+                             "rows_per_s": rows / (mlp_ms * 1e-3)},
     }
 
 

@@ -17,7 +17,7 @@ CP3H_FP64_PLAIN = 0
 CP3H_FP64 = 1
 CP3H_MIXED = 2
 PRECISIONS = {"fp64_plain": CP3H_FP64_PLAIN, "fp64": CP3H_FP64, "mixed": CP3H_MIXED}
-MLP_FORWARD_KERNELS = {"layered": 0, "fused4": 1, "fused8": 2}
+MLP_FORWARD_KERNELS = {"layered": 0, "fused4": 1, "fused8": 2, "fused64": 3}
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -75,6 +75,7 @@ SIGNATURES = {
     "cp3h_batch_set_binning": (ctypes.c_int, [_handle, ctypes.c_int, _c_float_p, ctypes.c_void_p,
                                               ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                                               _c_int_p]),
+    "cp3h_batch_set_row_limits": (ctypes.c_int, [_handle, ctypes.c_int, _c_int_p]),
     "cp3h_mlp_param_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t)]),
     "cp3h_mlp_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_handle)]),
     "cp3h_mlp_destroy": (ctypes.c_int, [_handle]),

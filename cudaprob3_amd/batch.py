@@ -155,6 +155,15 @@ class BatchPropagator:
                                                     ctypes.c_void_p(out_ptr or None), off.ctypes.data_as(llp),
                                                     ps.ctypes.data_as(llp), rs.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
 
+    def set_row_limits(self, i: int, limits):
+        """Row ic of set i computes only its first limits[ic] energies (None: all)."""
+        if limits is None:
+            _lib.check(self._lib.cp3h_batch_set_row_limits(self._h, i, None))
+            return
+        lim = np.ascontiguousarray(limits, np.int32)
+        assert lim.size == len(self.sets[i].cosines)
+        _lib.check(self._lib.cp3h_batch_set_row_limits(self._h, i, lim.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+
     def use_graph(self, enable: bool):
         _lib.check(self._lib.cp3h_batch_use_graph(self._h, int(bool(enable))))
 

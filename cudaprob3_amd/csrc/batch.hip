@@ -8,11 +8,19 @@
 //
 // Work items are (set, 64-energy tile, cosine row), numbered set-major, then tile-major, so
 // consecutive items share a spectrum-table slice.  A fixed grid of workgroups (sized from
-// the CU count by the autotuner) walks the item list: wavefront w of the logical grid takes
-// items w, w + W, w + 2W, ...; logical workgroup ids are laid out so that the wavefronts of
-// one XCD take consecutive items (same table slice in that XCD's L2).  Each wavefront is
-// independent: no barriers, and every wavefront leaves the loop once its next item is past
-// the end of the list.
+// the CU count by the autotuner) drains the item list through work queues, one per XCD:
+// the list is cut into kXcds contiguous parts, and the wavefronts of XCD x (workgroups are
+// dispatched to XCDs round robin, so x = blockIdx % kXcds) take kGrab consecutive items at a
+// time from part x with one atomic add on its counter; once part x is exhausted they help
+// with the other parts in turn.  So the items in flight on one XCD stay inside a narrow
+// window of its own part (a few table slices, resident in that XCD's L2) however much the
+// per-row cost varies (1 to 2 x shells layers per path), and the atomics spread over eight
+// counters.  (A static stride pins each wavefront to one cosine row when the stride is a
+// multiple of the row count, and the fast rows run ahead through the whole table; one
+// global queue needs large grabs to keep its atomic rate down, which widens the window past
+// L2 when sets are small.)  The counters are reset by the set-up kernel that precedes every
+// launch.  Each wavefront is independent: no barriers, and every wavefront leaves once a
+// grab from each of the kXcds parts has come back past that part's end.
 #include "common.hpp"
 #include "pair_mixed.hpp"
 
@@ -30,46 +38,80 @@ __device__ inline int item_set(const long long* __restrict__ off, int n, long lo
     return lo;
 }
 
+constexpr int kGrab = 4;
+
+// lane 0's value, held in scalar registers
+__device__ inline long long first_lane(unsigned long long v) {
+    const unsigned lo = unsigned(__builtin_amdgcn_readfirstlane(int(unsigned(v))));
+    const unsigned hi = unsigned(__builtin_amdgcn_readfirstlane(int(unsigned(v >> 32))));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// one work item: 64 energies (one per lane) of one cosine row (local = tile * n_c + row) of set k
+template <bool kBinned>
+__device__ inline void pair_item(const BatchDesc& b, int k, long long local, int lane) {
+    const BatchSet& set = b.sets[k];
+    const int tile = int(local / set.n_c);
+    const int ic = int(local % set.n_c);
+    const int ie = tile * kWave + lane;            // < ne_pad
+    const int n_e = set.has_limits ? min(set.n_e, b.row_limit[set.cos_off + ic]) : set.n_e;
+    if (tile * kWave >= n_e) return;               // the row's energies end before this tile
+    const int L = b.maxlayers[set.cos_off + ic];
+    const double* gl = b.geom + set.geom_off + std::size_t(ic) * (set.rows + 1);
+    float p[3][3];
+    propagate_pair_mixed(L, gl, b.Q + set.tab_off * 5, b.R + set.tab_off, set.ne_pad, ie, p);
+    if (kBinned) {
+        // groups of `nodes` lanes are whole bins, all inside or all beyond the set's n_e (a row
+        // limit inside a bin still leaves every node of that bin computed: the tile is whole)
+        const int nodes = b.bin_nodes;
+        const float w = b.bin_w[lane & (nodes - 1)];
+        float v[9];
+#pragma unroll
+        for (int t = 0; t < 9; t++) v[t] = w * p[t / 3][t % 3];
+        for (int off = nodes >> 1; off > 0; off >>= 1)
+#pragma unroll
+            for (int t = 0; t < 9; t++) v[t] += __shfl_xor(v[t], off);
+        if ((lane & (nodes - 1)) == 0 && ie < n_e) {
+            float* o = b.bin_out + set.bin_off + (long long)ic * set.bin_row + ie / nodes;
+#pragma unroll
+            for (int t = 0; t < 9; t++) o[t * set.bin_plane] = v[t];
+        }
+    } else if (ie < n_e) {
+        const std::size_t plane = std::size_t(set.n_c) * set.n_e;
+        float* o = b.out + set.out_off + std::size_t(ic) * set.n_e + ie;
+#pragma unroll
+        for (int in = 0; in < 3; in++)
+#pragma unroll
+            for (int ou = 0; ou < 3; ou++) o[(in * 3 + ou) * plane] = p[in][ou];
+    }
+}
+
 template <int kWaves, bool kBinned>
 __global__ __launch_bounds__(kWaves * kWave) void batch_pairs(BatchDesc b) {
     const long long total = b.item_prefix[b.n_sets];
-    const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / kWave);
     const int lane = int(threadIdx.x) % kWave;
-    const long long W = (long long)gridDim.x * kWaves;
-    for (long long item = (long long)lb * kWaves + wave; item < total; item += W) {
-        const int k = __builtin_amdgcn_readfirstlane(item_set(b.item_prefix, b.n_sets, item));
-        const BatchSet& set = b.sets[k];
-        const long long local = item - b.item_prefix[k];
-        const int tile = int(local / set.n_c);
-        const int ic = int(local % set.n_c);
-        const int ie = tile * kWave + lane;            // < ne_pad
-        const int L = b.maxlayers[set.cos_off + ic];
-        const double* gl = b.geom + set.geom_off + std::size_t(ic) * (set.rows + 1);
-        float p[3][3];
-        propagate_pair_mixed(L, gl, b.Q + set.tab_off * 5, b.R + set.tab_off, set.ne_pad, ie, p);
-        if (kBinned) {
-            // groups of `nodes` lanes are whole bins, all inside or all beyond n_e
-            const int nodes = b.bin_nodes;
-            const float w = b.bin_w[lane & (nodes - 1)];
-            float v[9];
-#pragma unroll
-            for (int t = 0; t < 9; t++) v[t] = w * p[t / 3][t % 3];
-            for (int off = nodes >> 1; off > 0; off >>= 1)
-#pragma unroll
-                for (int t = 0; t < 9; t++) v[t] += __shfl_xor(v[t], off);
-            if ((lane & (nodes - 1)) == 0 && ie < set.n_e) {
-                float* o = b.bin_out + set.bin_off + (long long)ic * set.bin_row + ie / nodes;
-#pragma unroll
-                for (int t = 0; t < 9; t++) o[t * set.bin_plane] = v[t];
+    const int home = int(blockIdx.x % kXcds);
+    for (int p = 0; p < kXcds; p++) {
+        const int part = (home + p) % kXcds;
+        const long long part_begin = total * part / kXcds, part_end = total * (part + 1) / kXcds;
+        unsigned long long* counter = b.next_item + part * kQueueStride;
+        for (;;) {
+            unsigned long long got = 0;
+            if (lane == 0) got = atomicAdd(counter, (unsigned long long)kGrab);
+            const long long first = part_begin + first_lane(got);
+            if (first >= part_end) break;
+            const long long last = first + kGrab < part_end ? first + kGrab : part_end;
+            // the set of the grab's first item, then step forward across set boundaries
+            int k = __builtin_amdgcn_readfirstlane(item_set(b.item_prefix, b.n_sets, first));
+            long long begin = b.item_prefix[k], end = b.item_prefix[k + 1];
+            for (long long item = first; item < last; item++) {
+                while (item >= end) {
+                    k++;
+                    begin = end;
+                    end = b.item_prefix[k + 1];
+                }
+                pair_item<kBinned>(b, k, item - begin, lane);
             }
-        } else if (ie < set.n_e) {
-            const std::size_t plane = std::size_t(set.n_c) * set.n_e;
-            float* o = b.out + set.out_off + std::size_t(ic) * set.n_e + ie;
-#pragma unroll
-            for (int in = 0; in < 3; in++)
-#pragma unroll
-                for (int ou = 0; ou < 3; ou++) o[(in * 3 + ou) * plane] = p[in][ou];
         }
     }
 }

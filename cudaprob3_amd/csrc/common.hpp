@@ -110,10 +110,13 @@ struct BatchSet {
     // binned mode: bin average (t, ic, bin) -> bin_out[bin_off + t * bin_plane + ic * bin_row + bin]
     long long bin_off, bin_plane;
     int bin_row;
+    int has_limits;       // row ic computes only energies below row_limit[cos_off + ic]
     int n_c, n_e, rows, ne_pad;
 };
 
 constexpr int kMaxBinNodes = 32;
+// work-queue counters of the batched pair kernel, one per XCD, each in its own 256-byte line
+constexpr int kQueueStride = 32;
 
 struct BatchDesc {
     const BatchSet* sets;
@@ -129,6 +132,9 @@ struct BatchDesc {
     double* geom;
     float4* Q;
     double2* R;
+    unsigned long long* next_item;  // kXcds work-queue counters of the pair kernel, kQueueStride apart
+                                    // (zeroed by the set-up kernel)
+    const int* row_limit;           // per cosine row (parallel to cosines): energies to compute
     float* out;                     // unbinned results (bin_nodes == 0)
     float* bin_out;                 // binned mode: destination of the bin averages
     int bin_nodes;                  // 0, or quadrature nodes per energy bin (a power of two <= 32)

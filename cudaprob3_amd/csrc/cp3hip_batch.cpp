@@ -37,6 +37,9 @@ struct cp3h_batch {
     double* d_geom = nullptr;
     float4* d_Q = nullptr;
     double2* d_R = nullptr;
+    unsigned long long* d_next = nullptr;   // pair-kernel work queue
+    int* d_limits = nullptr;                // per-row energy limits, parallel to the cosines
+    long long cos_total = 0;
     float* d_out = nullptr;             // unbinned results, allocated on first use
     long long out_n = 0;
     float* bin_out = nullptr;           // binned mode
@@ -52,7 +55,8 @@ struct cp3h_batch {
     cp3::BatchDesc desc() const {
         const long long n1 = n_sets + 1;
         cp3::BatchDesc d{d_sets, n_sets, d_prefix, d_prefix + n1, d_prefix + 2 * n1, d_cos, d_maxlayers,
-                         d_energies, d_radii, d_rhos, d_geom, d_Q, d_R, d_out, bin_out, bin_nodes, {}};
+                         d_energies, d_radii, d_rhos, d_geom, d_Q, d_R, d_next, d_limits, d_out, bin_out,
+                         bin_nodes, {}};
         for (int g = 0; g < cp3::kMaxBinNodes; g++) d.bin_w[g] = bin_w[g];
         return d;
     }
@@ -206,6 +210,8 @@ int cp3h_batch_create(int device, int n_sets, const int* n_cosines, const int* n
         CP3H_HIP(hipMalloc(&b->d_geom, sizeof(double) * std::max(geom_n, 1LL)));
         CP3H_HIP(hipMalloc(&b->d_Q, sizeof(float4) * 5 * std::max(tab_n, 1LL)));
         CP3H_HIP(hipMalloc(&b->d_R, sizeof(double2) * std::max(tab_n, 1LL)));
+        CP3H_HIP(hipMalloc(&b->d_next, sizeof(unsigned long long) * cp3::kXcds * cp3::kQueueStride));
+        b->cos_total = cos_n;
         b->out_n = out_n;
         std::vector<long long> prefix;
         prefix.insert(prefix.end(), b->geom_prefix.begin(), b->geom_prefix.end());
@@ -233,6 +239,8 @@ int cp3h_batch_destroy(cp3h_batch* b) {
     free_dev(b->d_geom);
     free_dev(b->d_Q);
     free_dev(b->d_R);
+    free_dev(b->d_next);
+    free_dev(b->d_limits);
     free_dev(b->d_out);
     if (b->pinned) (void)hipHostFree(b->pinned);
     if (b->ev_start) (void)hipEventDestroy(b->ev_start);
@@ -446,6 +454,26 @@ int cp3h_batch_device_results(cp3h_batch* b, int set, const float** ptr) {
     if (!ptr) return fail(CP3H_ERR_ARGUMENT, "null output pointer");
     if (int rc = ensure_out(b)) return rc;
     *ptr = b->d_out + b->layout[set].out_off;
+    return CP3H_OK;
+}
+
+int cp3h_batch_set_row_limits(cp3h_batch* b, int set, const int* limits) {
+    if (int rc = check(b, set)) return rc;
+    const cp3::BatchSet& s = b->layout[set];
+    if (limits)
+        for (int i = 0; i < s.n_c; i++)
+            if (limits[i] < 0 || limits[i] > s.n_e)
+                return fail(CP3H_ERR_ARGUMENT, "row limits must lie in [0, n_energies]");
+    if (int rc = drain(b)) return rc;
+    b->launched = false;
+    if (limits) {
+        if (!b->d_limits) {
+            CP3H_HIP(hipMalloc(&b->d_limits, sizeof(int) * std::max(b->cos_total, 1LL)));
+            drop_graph(b);      // the kernel arguments now carry the limit array
+        }
+        if (int rc = upload(b, b->d_limits + s.cos_off, limits, sizeof(int) * s.n_c)) return rc;
+    }
+    b->pinned[set].has_limits = limits ? 1 : 0;
     return CP3H_OK;
 }
 

@@ -70,7 +70,7 @@ struct cp3h_mlp {
     float* loss_host = nullptr;
     cp3::MlpScaling scaling{};
     int adam_t = 0;
-    int forward_kernel = CP3H_MLP_FUSED8;
+    int forward_kernel = CP3H_MLP_FUSED4;
     std::vector<void*> allocations;
 };
 
@@ -320,8 +320,8 @@ int cp3h_mlp_forward(cp3h_mlp* h, const float* x_device, long long n, float* y_d
     if (n < 0 || (n > 0 && (!x_device || !y_device))) return fail(CP3H_ERR_ARGUMENT, "bad forward arguments");
     CP3H_HIP(hipSetDevice(h->device));
     if (h->forward_kernel != CP3H_MLP_LAYERED) {
-        CP3H_HIP(cp3::launch_mlp_forward_fused(x_device, n, y_device, h->wb, h->b, h->scaling,
-                                               h->forward_kernel == CP3H_MLP_FUSED8 ? 2 : 1, nullptr, h->stream));
+        CP3H_HIP(cp3::launch_mlp_forward_fused(x_device, n, y_device, h->wb, h->b, h->scaling, h->forward_kernel,
+                                               nullptr, h->stream));
         return CP3H_OK;
     }
     for (long long r0 = 0; r0 < n; r0 += h->max_batch) {
@@ -341,14 +341,15 @@ int cp3h_mlp_forward_grid(cp3h_mlp* h, const float* cos_device, int n_cos, const
         return fail(CP3H_ERR_ARGUMENT, "null grid arrays");
     CP3H_HIP(hipSetDevice(h->device));
     const cp3::MlpGrid g{cos_device, log10e_device, hyp_device, skip_below_device, n_cos, n_bins};
-    CP3H_HIP(cp3::launch_mlp_forward_fused(nullptr, n, y_device, h->wb, h->b, h->scaling,
-                                           h->forward_kernel == CP3H_MLP_FUSED4 ? 1 : 2, &g, h->stream));
+    const int variant = h->forward_kernel == CP3H_MLP_LAYERED ? CP3H_MLP_FUSED4 : h->forward_kernel;
+    CP3H_HIP(cp3::launch_mlp_forward_fused(nullptr, n, y_device, h->wb, h->b, h->scaling, variant, &g, h->stream));
     return CP3H_OK;
 }
 
 int cp3h_mlp_set_forward_kernel(cp3h_mlp* h, int kernel) {
     if (int rc = check(h)) return rc;
-    if (kernel != CP3H_MLP_LAYERED && kernel != CP3H_MLP_FUSED4 && kernel != CP3H_MLP_FUSED8)
+    if (kernel != CP3H_MLP_LAYERED && kernel != CP3H_MLP_FUSED4 && kernel != CP3H_MLP_FUSED8 &&
+        kernel != CP3H_MLP_FUSED64)
         return fail(CP3H_ERR_ARGUMENT, "unknown forward kernel");
     h->forward_kernel = kernel;
     return CP3H_OK;

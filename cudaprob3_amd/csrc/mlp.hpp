@@ -77,10 +77,11 @@ struct MlpGrid {
     int n_cos, n_bins;
 };
 
-// all layers of inference in one launch (mlp_infer.hip); row_groups 1 or 2 (4 or 8 wavefronts);
+// all layers of inference in one launch (mlp_infer.hip), tile shape by variant;
 // grid: null for row mode (x [n][9] -> y [n][9]), else grid mode (y [n_hyp][9][n_cos][n_bins])
+enum FusedVariant { kFusedRows128Waves4 = 1, kFusedRows128Waves8 = 2, kFusedRows64 = 3 };
 hipError_t launch_mlp_forward_fused(const float* x, long long n, float* y, const __bf16* const* w,
-                                    const float* const* b, const MlpScaling& sc, int row_groups, const MlpGrid* grid,
+                                    const float* const* b, const MlpScaling& sc, int variant, const MlpGrid* grid,
                                     hipStream_t s);
 hipError_t launch_adam_vec(float* w, float* m1, float* m2, const float* g, int n, int n_valid, const AdamStep& a,
                            hipStream_t s);

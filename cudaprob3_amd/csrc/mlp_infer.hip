@@ -1,6 +1,6 @@
-// Fused inference of the surrogate network: one launch runs all six layers over a tile of 128
-// rows, with the activations resident in LDS between layers (no HBM traffic but the 36-byte
-// input and output rows).
+// Fused inference of the surrogate network: one launch runs all six layers over a tile of 64 or
+// 128 rows, with the activations resident in LDS between layers (no HBM traffic but the
+// 36-byte input and output rows).
 //
 // LDS holds the tile's activations as two-term bf16 (hi in columns 0..255, lo in 256..511;
 // before the first layer, the scaled inputs' hi / lo in columns 0..31), rows padded to 520
@@ -15,8 +15,10 @@
 // registers until every wavefront has finished reading the layer's inputs (workgroup barrier),
 // then overwrite them in place.
 //
-// kRowGroups = 1: 4 wavefronts, each over all 128 rows; kRowGroups = 2: 8 wavefronts, row
-// halves x 4 feature slices.
+// Variants: 128-row tiles with 4 wavefronts (each over all rows) or 8 (row halves x 4 feature
+// slices), one workgroup per CU (133 KB of LDS); 64-row tiles with 4 wavefronts and 66.5 KB, two
+// workgroups per CU whose layers drift out of phase, so one's GELU epilogue (vector ALU) runs
+// beside the other's MFMAs.
 //
 // Grid mode (kGrid): the rows are the points (hypothesis h, cosine ic, energy bin ib) of
 // n_hyp hypotheses on one cosZ x bin grid, h-major then ic then ib; the kernel forms each
@@ -32,7 +34,6 @@
 namespace cp3 {
 namespace {
 
-constexpr int kTileRows = 128;
 constexpr int kActStride = kMlpActLd + 8;
 
 struct FusedArgs {
@@ -45,29 +46,20 @@ struct FusedArgs {
     MlpGrid g;                         // grid mode
 };
 
-// grid mode: hypothesis and in-plane index of a row
-struct GridRow {
-    long long h;
-    int rr;      // ic * n_bins + ib
+// grid mode: (hypothesis, cosine, bin) of each row of the tile, computed once per tile (one
+// 64-bit division per thread) and kept in LDS for the input and output stages
+template <int kTileRows>
+struct TileRows {
+    int h[kTileRows], ic[kTileRows], ib[kTileRows];
 };
-
-__device__ inline GridRow grid_row(const MlpGrid& g, long long row) {
-    const long long plane = (long long)g.n_cos * g.n_bins;
-    const long long h = row / plane;
-    return GridRow{h, int(row - h * plane)};
-}
-
-__device__ inline unsigned bf16_pair(__bf16 lo_half, __bf16 hi_half) {
-    return unsigned(__builtin_bit_cast(unsigned short, lo_half)) |
-           (unsigned(__builtin_bit_cast(unsigned short, hi_half)) << 16);
-}
 
 // One GELU layer over the tile: act (hi/lo) -> act (hi/lo).  kFirst: the 9 -> 256 layer, whose
 // K = 32 columns already hold both terms of the inputs (one MFMA per feature block).
-template <int kRowGroups, bool kFirst>
-__device__ inline void hidden_layer(__bf16* act, const __bf16* __restrict__ w, const float* __restrict__ bias,
-                                    int wm, int wn, int lane) {
-    constexpr int kMT = 8 / kRowGroups;                    // 16-row tiles of this wavefront
+template <int kTileRows, int kRowGroups, bool kFirst>
+__device__ __attribute__((always_inline)) inline void hidden_layer(__bf16* act, const __bf16* __restrict__ w,
+                                                                   const float* __restrict__ bias, int wm, int wn,
+                                                                   int lane) {
+    constexpr int kMT = kTileRows / 16 / kRowGroups;       // 16-row tiles of this wavefront
     constexpr int kKS = kFirst ? 1 : kMlpHidden / 32;      // k-steps over one term
     constexpr int kLd = kFirst ? kMlpInPad : kMlpActLd;    // row length of w
     const int l16 = lane % 16, q = lane / 16;
@@ -103,15 +95,11 @@ __device__ inline void hidden_layer(__bf16* act, const __bf16* __restrict__ w, c
         }
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            __bf16 h[4], lo[4];
+            float g[4];
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const float g = gelu(acc[j][r] + bv[j][r]);
-                h[r] = __bf16(g);
-                lo[r] = __bf16(g - float(h[r]));
-            }
-            out_hi[i][j] = uint2{bf16_pair(h[0], h[1]), bf16_pair(h[2], h[3])};
-            out_lo[i][j] = uint2{bf16_pair(lo[0], lo[1]), bf16_pair(lo[2], lo[3])};
+            for (int r = 0; r < 4; r++) g[r] = gelu(acc[j][r] + bv[j][r]);
+            two_term_pair(g[0], g[1], out_hi[i][j].x, out_lo[i][j].x);
+            two_term_pair(g[2], g[3], out_hi[i][j].y, out_lo[i][j].y);
         }
     }
     __syncthreads();    // every wavefront has read this layer's inputs
@@ -128,10 +116,11 @@ __device__ inline void hidden_layer(__bf16* act, const __bf16* __restrict__ w, c
 }
 
 // 256 -> 9 linear layer (16 computed features, 9 stored); the wavefronts take 16-row tiles in turn.
-template <int kRowGroups, bool kGrid>
-__device__ inline void output_layer(const __bf16* act, const __bf16* __restrict__ w, const float* __restrict__ bias,
-                                    float* __restrict__ y, long long row0, long long n, const MlpGrid& grid, int wave,
-                                    int lane) {
+template <int kTileRows, int kRowGroups, bool kGrid>
+__device__ __attribute__((always_inline)) inline void output_layer(const __bf16* act, const __bf16* __restrict__ w,
+                                                                   const float* __restrict__ bias, float* __restrict__ y,
+                                                                   long long row0, long long n, const MlpGrid& grid,
+                                                                   const TileRows<kTileRows>& tr, int wave, int lane) {
     constexpr int kWaves = 4 * kRowGroups;
     constexpr int kKS = kMlpHidden / 32;
     const int l16 = lane % 16, q = lane / 16;
@@ -153,11 +142,11 @@ __device__ inline void output_layer(const __bf16* act, const __bf16* __restrict_
         const long long row = row0 + i * 16 + l16;         // lane holds features 4q .. 4q+3 of this row
         if (row < n) {
             if (kGrid) {
-                const GridRow gr = grid_row(grid, row);
+                const int r16 = i * 16 + l16;
+                const int h = tr.h[r16], ic = tr.ic[r16], ib = tr.ib[r16];
                 const long long plane = (long long)grid.n_cos * grid.n_bins;
-                const int ic = gr.rr / grid.n_bins, ib = gr.rr % grid.n_bins;
-                if (!grid.skip_below || ib >= grid.skip_below[gr.h * grid.n_cos + ic]) {
-                    float* o = y + gr.h * kMlpOut * plane + gr.rr;
+                if (!grid.skip_below || ib >= grid.skip_below[(long long)h * grid.n_cos + ic]) {
+                    float* o = y + (long long)h * kMlpOut * plane + (long long)ic * grid.n_bins + ib;
 #pragma unroll
                     for (int r = 0; r < 4; r++)
                         if (4 * q + r < kMlpOut) o[(4 * q + r) * plane] = acc[r] + bv[r];
@@ -171,18 +160,25 @@ __device__ inline void output_layer(const __bf16* act, const __bf16* __restrict_
     }
 }
 
-template <int kRowGroups, bool kGrid>
+template <int kTileRows, int kRowGroups, bool kGrid>
 __global__ __launch_bounds__(256 * kRowGroups) void mlp_forward_fused(FusedArgs a) {
     __shared__ __bf16 act[kTileRows * kActStride];
+    __shared__ TileRows<kTileRows> tr;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const long long row0 = (long long)blockIdx.x * kTileRows;
 
-    if (kGrid && a.g.skip_below) {
+    if (kGrid) {
         int needed = 0;
-        if (threadIdx.x < kTileRows && row0 + threadIdx.x < a.n) {
-            const GridRow gr = grid_row(a.g, row0 + threadIdx.x);
-            const int ic = gr.rr / a.g.n_bins, ib = gr.rr % a.g.n_bins;
-            needed = ib >= a.g.skip_below[gr.h * a.g.n_cos + ic];
+        if (threadIdx.x < kTileRows) {
+            const long long row = row0 + threadIdx.x;
+            const long long plane = (long long)a.g.n_cos * a.g.n_bins;
+            const long long h = row / plane;
+            const int rr = int(row - h * plane);
+            const int ic = rr / a.g.n_bins, ib = rr - ic * a.g.n_bins;
+            tr.h[threadIdx.x] = int(h);
+            tr.ic[threadIdx.x] = ic;
+            tr.ib[threadIdx.x] = ib;
+            needed = row < a.n && (!a.g.skip_below || ib >= a.g.skip_below[h * a.g.n_cos + ic]);
         }
         if (!__syncthreads_or(needed)) return;      // the whole workgroup leaves together
     }
@@ -195,10 +191,9 @@ __global__ __launch_bounds__(256 * kRowGroups) void mlp_forward_fused(FusedArgs 
         if (row0 + r < a.n && src < kMlpIn) {
             float x;
             if (kGrid) {
-                const GridRow gr = grid_row(a.g, row0 + r);
-                x = src == 0   ? a.g.cos[gr.rr / a.g.n_bins]
-                    : src == 1 ? a.g.log10e[gr.rr % a.g.n_bins]
-                               : a.g.hyp[gr.h * (kMlpIn - 2) + src - 2];
+                x = src == 0   ? a.g.cos[tr.ic[r]]
+                    : src == 1 ? a.g.log10e[tr.ib[r]]
+                               : a.g.hyp[(long long)tr.h[r] * (kMlpIn - 2) + src - 2];
             } else {
                 x = a.x[(row0 + r) * kMlpIn + src];
             }
@@ -210,16 +205,18 @@ __global__ __launch_bounds__(256 * kRowGroups) void mlp_forward_fused(FusedArgs 
     __syncthreads();
 
     const int wm = wave / 4, wn = wave % 4;
-    hidden_layer<kRowGroups, true>(act, a.w[0], a.b[0], wm, wn, lane);
+    hidden_layer<kTileRows, kRowGroups, true>(act, a.w[0], a.b[0], wm, wn, lane);
 #pragma unroll 1
-    for (int l = 1; l < kMlpHiddenLayers; l++) hidden_layer<kRowGroups, false>(act, a.w[l], a.b[l], wm, wn, lane);
-    output_layer<kRowGroups, kGrid>(act, a.w[kMlpLayers - 1], a.b[kMlpLayers - 1], a.y, row0, a.n, a.g, wave, lane);
+    for (int l = 1; l < kMlpHiddenLayers; l++)
+        hidden_layer<kTileRows, kRowGroups, false>(act, a.w[l], a.b[l], wm, wn, lane);
+    output_layer<kTileRows, kRowGroups, kGrid>(act, a.w[kMlpLayers - 1], a.b[kMlpLayers - 1], a.y, row0, a.n, a.g, tr,
+                                               wave, lane);
 }
 
 }  // namespace
 
 hipError_t launch_mlp_forward_fused(const float* x, long long n, float* y, const __bf16* const* w,
-                                    const float* const* b, const MlpScaling& sc, int row_groups, const MlpGrid* grid,
+                                    const float* const* b, const MlpScaling& sc, int variant, const MlpGrid* grid,
                                     hipStream_t s) {
     if (n <= 0) return hipSuccess;
     FusedArgs a{x, n, y, {}, {}, sc, grid ? *grid : MlpGrid{}};
@@ -227,16 +224,20 @@ hipError_t launch_mlp_forward_fused(const float* x, long long n, float* y, const
         a.w[l] = w[l];
         a.b[l] = b[l];
     }
-    const long long blocks = (n + kTileRows - 1) / kTileRows;
-    if (blocks > 0x7fffffffLL || (grid && (grid->n_cos <= 0 || grid->n_bins <= 0))) return hipErrorInvalidValue;
-#define CP3_FUSED(G, GRID) \
-    hipLaunchKernelGGL((mlp_forward_fused<G, GRID>), dim3(unsigned(blocks)), dim3(256 * G), 0, s, a)
-    if (row_groups == 1) {
-        if (grid) CP3_FUSED(1, true); else CP3_FUSED(1, false);
-    } else if (row_groups == 2) {
-        if (grid) CP3_FUSED(2, true); else CP3_FUSED(2, false);
-    } else {
-        return hipErrorInvalidValue;
+    if (grid && (grid->n_cos <= 0 || grid->n_bins <= 0)) return hipErrorInvalidValue;
+    const int tile = variant == kFusedRows64 ? 64 : 128;
+    const long long blocks = (n + tile - 1) / tile;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+#define CP3_FUSED(T, G)                                                                                      \
+    do {                                                                                                     \
+        if (grid) hipLaunchKernelGGL((mlp_forward_fused<T, G, true>), dim3(unsigned(blocks)), dim3(256 * G), 0, s, a); \
+        else hipLaunchKernelGGL((mlp_forward_fused<T, G, false>), dim3(unsigned(blocks)), dim3(256 * G), 0, s, a); \
+    } while (0)
+    switch (variant) {
+        case kFusedRows128Waves4: CP3_FUSED(128, 1); break;
+        case kFusedRows128Waves8: CP3_FUSED(128, 2); break;
+        case kFusedRows64: CP3_FUSED(64, 1); break;
+        default: return hipErrorInvalidValue;
     }
 #undef CP3_FUSED
     return hipGetLastError();

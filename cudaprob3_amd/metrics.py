@@ -43,3 +43,8 @@ def maxlayers_of(cosines, radii, rhos):
     p.setCosineList(cosines)
     p.setDensity(radii, rhos)
     return p.maxlayers.copy()
+
+
+# Surrogate network, algorithmic flops per evaluated row: the forward products with two-term
+# (hi + lo) inputs and activations, without padding: 2 * (18*256 + 4*512*256 + 512*9).
+MLP_FLOPS_PER_ROW = 2 * (18 * 256 + 4 * 512 * 256 + 512 * 9)

@@ -99,7 +99,7 @@ class SurrogateNet:
         return p
 
     def set_forward_kernel(self, name: str):
-        """'fused8' (default), 'fused4' or 'layered' (the training path's per-layer GEMMs)."""
+        """'fused4' (default), 'fused8', 'fused64' or 'layered' (the training path's per-layer GEMMs)."""
         _lib.check(self._lib.cp3h_mlp_set_forward_kernel(self._h, _lib.MLP_FORWARD_KERNELS[name]))
 
     def set_stream(self, stream_ptr):
@@ -318,9 +318,11 @@ def _hyp_row(params, antinu):
 class GridEvaluator:
     """Bin-averaged probabilities of many hypotheses on one cosZ x energy-bin grid (the jittered
     12-shell Earth): the network fills the grid (fused kernel, grid mode) except, per cosine row,
-    the leading run of energy bins whose error-map cells exceed tol / safety; the exact
-    mixed-precision kernel computes those bins (batched, binned mode: 16 Gauss-Legendre nodes
-    per bin, averaged in registers) into the same array at the same time, on its own stream.
+    the leading run of energy bins up to the last one whose error-map cell exceeds tol / safety;
+    the exact mixed-precision kernel computes those bins (batched, binned mode: 16
+    Gauss-Legendre nodes per bin, averaged in registers; one set per hypothesis with per-row
+    energy limits, so one spectrum table per hypothesis) into the same array, on its own
+    stream (the two kernels write disjoint bins).
 
     Hypotheses are processed `slots` at a time per class; results go to ``self.out``
     [slots][9][n_cos][n_bins] (t = in*3 + out) and are handed to ``consume``.
@@ -342,29 +344,23 @@ class GridEvaluator:
         self.out = torch.empty((slots, 9, n_c, n_b), dtype=torch.float32, device=dev)
         radii, rhos = W.jitter12_model()
         E_nodes, self.weights = gl_energies(self.centers)
-        self.skip, self.skip_t, self.batches, self.runs = [], [], [], []
+        self.skip, self.skip_t, self.batches = [], [], []
         for k in range(N_CLASSES):
             mask = emap.fallback(k, self.cos, l10, tol, safety)
             last = np.where(mask.any(axis=1), n_b - np.argmax(mask[:, ::-1], axis=1), 0)   # prefix lengths
             self.skip.append(last.astype(np.int32))
             self.skip_t.append(torch.as_tensor(np.tile(last.astype(np.int32), (slots, 1)), device=dev))
-            runs = []          # consecutive cosine rows with the same prefix -> one exact set
-            c0 = 0
-            for c in range(1, n_c + 1):
-                if c == n_c or last[c] != last[c0]:
-                    if last[c0] > 0:
-                        runs.append((c0, c, int(last[c0])))
-                    c0 = c
-            self.runs.append(runs)
-            if not runs:
+            kmax = int(last.max())
+            if kmax == 0:
                 self.batches.append(None)
                 continue
             dummy = (0.59, 0.15, 0.78, 0.0, 7.4e-5, 2.5e-3 if k % 2 == 0 else -2.5e-3)
-            specs = [SetSpec(self.cos[a:b], E_nodes[:16 * kb], radii, rhos, dummy, k // 2)
-                     for _ in range(slots) for (a, b, kb) in runs]
+            specs = [SetSpec(self.cos, E_nodes[:16 * kmax], radii, rhos, dummy, k // 2) for _ in range(slots)]
             bp = BatchPropagator(specs, device=device, graph=True)
-            offs = [s * 9 * self.plane + a * n_b for s in range(slots) for (a, _, _) in runs]
-            bp.set_binning(16, self.weights, self.out.data_ptr(), offs, self.plane, n_b)
+            for i in range(slots):
+                bp.set_row_limits(i, 16 * last)
+            bp.set_binning(16, self.weights, self.out.data_ptr(), [i * 9 * self.plane for i in range(slots)],
+                           self.plane, n_b)
             self.batches.append(bp)
         torch.cuda.synchronize(dev)
 
@@ -391,9 +387,7 @@ class GridEvaluator:
             net.forward_grid(self.cos_t, self.l10_t, hyp, self.skip_t[k], self.out)
             bp = self.batches[k]
             if bp is not None:
-                runs = self.runs[k]
-                p = np.repeat(np.array([r[:6] for r in rows], np.float64), len(runs), axis=0)
-                bp.set_params_many(p, np.full(len(p), k // 2, np.int32))
+                bp.set_params_many(np.array([r[:6] for r in rows], np.float64), np.full(self.slots, k // 2, np.int32))
                 bp.compute_async()
             net.sync()
             if bp is not None:

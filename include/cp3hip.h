@@ -196,6 +196,12 @@ int cp3h_batch_bin_average_f32(cp3h_batch* b, int first, int count, int nodes, c
 int cp3h_batch_set_binning(cp3h_batch* b, int nodes, const float* weights, float* out_device,
                            const long long* offsets, const long long* plane_strides, const int* row_strides);
 
+/* Per-row energy limits (no counterpart in the reference): cosine row ic of `set` computes and
+   stores only its first limits[ic] energies (in binned mode: the bins starting below it);
+   limits = NULL removes them.  Lets one set (one spectrum table) cover rows that need different
+   energy ranges. */
+int cp3h_batch_set_row_limits(cp3h_batch* b, int set, const int* limits);
+
 /* ---- Surrogate network --------------------------------------------------------------------
  * No counterpart in the reference.  A 9 -> 256 x 5 (GELU, tanh form) -> 9 MLP with BF16
  * weights and FP32 accumulation on the matrix cores, approximating energy-bin-averaged
@@ -220,9 +226,10 @@ int cp3h_mlp_get_params(cp3h_mlp* h, float* params, size_t n);
 /* inference of n rows (asynchronous on the network's stream) */
 int cp3h_mlp_forward(cp3h_mlp* h, const float* x_device, long long n, float* y_device);
 /* inference kernel: CP3H_MLP_FUSED4 / CP3H_MLP_FUSED8 run all layers in one launch per 128
-   rows (4 or 8 wavefronts per workgroup; FUSED8 is the default); CP3H_MLP_LAYERED runs the
-   training path's per-layer GEMMs in chunks of max_batch rows. */
-enum { CP3H_MLP_LAYERED = 0, CP3H_MLP_FUSED4 = 1, CP3H_MLP_FUSED8 = 2 };
+   rows (4 or 8 wavefronts per workgroup), CP3H_MLP_FUSED64 per 64 rows (two workgroups per CU);
+   FUSED4 is the default; CP3H_MLP_LAYERED runs the training path's per-layer GEMMs in chunks of
+   max_batch rows (row mode only; grid mode then uses FUSED4). */
+enum { CP3H_MLP_LAYERED = 0, CP3H_MLP_FUSED4 = 1, CP3H_MLP_FUSED8 = 2, CP3H_MLP_FUSED64 = 3 };
 int cp3h_mlp_set_forward_kernel(cp3h_mlp* h, int kernel);
 /* Inference on n_hyp hypotheses over one cosZ x energy-bin grid (fused kernel): the inputs
    of point (h, ic, ib) are cos[ic], log10e[ib] (log10 of the bin centre / GeV) and hyp[h][0..6]

@@ -167,3 +167,44 @@ def test_binned_mode_rejects_bad_layouts():
         b.set_binning(0, None, 0, 0, 0, 0)                              # back to unbinned results
         b.compute()
         assert b.results(0).shape == (9, 2, 32)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_row_limits(graph):
+    """Per-row energy limits: each row's leading energies (bins in binned mode) equal the
+    unlimited computation bit for bit; nothing past a row's limit is written."""
+    import torch
+    from cudaprob3_amd.surrogate import gl_energies
+    rng = np.random.Generator(np.random.PCG64(12))
+    radii, rhos = W.jitter12_model()
+    cos = np.linspace(-0.98, 0.97, 37)
+    centers = np.geomspace(0.07, 700, 21)
+    E, w = gl_energies(centers)
+    sets = [SetSpec(cos, E, radii, rhos, W.random_params(rng), antinu=i) for i in range(2)]
+    lim_bins = rng.integers(0, 22, (2, len(cos))).astype(np.int32)
+    lim_bins[0, 3] = 0
+    lim_bins[1, 5] = 21
+    print(f"[gpu] batch row limits, graph={graph}", flush=True)
+    with BatchPropagator(sets, graph=False) as b:
+        b.compute()
+        full = [b.results(i) for i in range(2)]
+    nb = len(centers)
+    out = torch.full((2, 9, len(cos), nb), -5.0, device="cuda")
+    with BatchPropagator(sets, graph=graph) as b:
+        for i in range(2):
+            b.set_row_limits(i, 16 * lim_bins[i])
+        torch.cuda.synchronize()
+        b.set_binning(16, w, out.data_ptr(), [0, 9 * len(cos) * nb], len(cos) * nb, nb)
+        b.compute()
+        b.compute()
+        got = out.cpu().numpy()
+        b.set_binning(0, None, 0, 0, 0, 0)
+        b.compute()
+        limited = [b.results(i) for i in range(2)]
+    for i in range(2):
+        keep = np.arange(nb)[None, :] < lim_bins[i][:, None]                        # [ic][bin]
+        avg = np.einsum("tcbg,g->tcb", full[i].reshape(9, len(cos), nb, 16).astype(np.float64), w.astype(np.float64))
+        assert np.abs(got[i][:, keep] - avg[:, keep]).max() <= 2e-6
+        assert np.all(got[i][:, ~keep] == -5.0)
+        keep_e = np.repeat(keep, 16, axis=1)
+        np.testing.assert_array_equal(limited[i][:, keep_e], full[i][:, keep_e])

@@ -26,7 +26,7 @@ def batch(torch, M, seed):
     return x, t, torch.as_tensor(x, device="cuda"), torch.as_tensor(t, device="cuda")
 
 
-@pytest.mark.parametrize("kernel", ["layered", "fused4", "fused8"])
+@pytest.mark.parametrize("kernel", ["layered", "fused4", "fused8", "fused64"])
 def test_forward_matches_oracle(torch, kernel):
     """Two-term activations: the kernels agree with the float64 network (same bf16 weights,
     hi + lo inputs and activations) to FP32 accumulation error."""
@@ -51,11 +51,12 @@ def test_fused_forward_large_and_empty(torch):
     net = SurrogateNet(max_batch=8192, training=False, seed=4)
     _, _, xd, _ = batch(torch, 100_003, 5)
     out = {}
-    for k in ("layered", "fused4", "fused8"):
+    for k in ("layered", "fused4", "fused8", "fused64"):
         net.set_forward_kernel(k)
         out[k] = net.forward(xd).cpu().numpy()
     assert np.abs(out["fused4"] - out["layered"]).max() < 1e-4
     assert np.abs(out["fused8"] - out["fused4"]).max() < 1e-4
+    assert np.abs(out["fused64"] - out["fused4"]).max() < 1e-4
     empty = torch.empty((0, 9), dtype=torch.float32, device="cuda")
     assert net.forward(empty).shape == (0, 9)
     net.close()
@@ -146,7 +147,7 @@ def test_forward_grid_matches_row_mode(torch):
     skip = np.random.default_rng(1).integers(0, n_b + 1, (n_hyp, n_c)).astype(np.int32)
     skip[1] = n_b                                    # hypothesis 1: everything skipped
     dev = lambda a: torch.as_tensor(a, device="cuda")
-    for kernel in ("fused4", "fused8"):
+    for kernel in ("fused4", "fused8", "fused64"):
         net.set_forward_kernel(kernel)
         out = torch.full((n_hyp, 9, n_c, n_b), -3.0, device="cuda")
         net.forward_grid(dev(cos), dev(l10), dev(hyp), None, out)
