@@ -47,7 +47,7 @@ std::size_t param_count() {
 }  // namespace
 
 struct cp3h_mlp {
-    int device = 0, max_batch = 0;
+    int device = 0, max_batch = 0, cu_count = 1;
     bool training = false;
     hipStream_t stream = nullptr, own_stream = nullptr;
     float* w[cp3::kMlpLayers] = {};
@@ -188,6 +188,7 @@ int cp3h_mlp_create(int device, int max_batch, int training, cp3h_mlp** out) {
     for (int c = 0; c < cp3::kMlpIn; c++) h->scaling.scale[c] = 1.0f;
     const int rc = [&]() -> int {
         CP3H_HIP(hipSetDevice(device));
+        CP3H_HIP(hipDeviceGetAttribute(&h->cu_count, hipDeviceAttributeMultiprocessorCount, device));
         CP3H_HIP(hipStreamCreate(&h->own_stream));
         h->stream = h->own_stream;
         const std::size_t M = max_batch;
@@ -321,7 +322,7 @@ int cp3h_mlp_forward(cp3h_mlp* h, const float* x_device, long long n, float* y_d
     CP3H_HIP(hipSetDevice(h->device));
     if (h->forward_kernel != CP3H_MLP_LAYERED) {
         CP3H_HIP(cp3::launch_mlp_forward_fused(x_device, n, y_device, h->wb, h->b, h->scaling, h->forward_kernel,
-                                               nullptr, h->stream));
+                                               nullptr, h->cu_count, h->stream));
         return CP3H_OK;
     }
     for (long long r0 = 0; r0 < n; r0 += h->max_batch) {
@@ -342,7 +343,8 @@ int cp3h_mlp_forward_grid(cp3h_mlp* h, const float* cos_device, int n_cos, const
     CP3H_HIP(hipSetDevice(h->device));
     const cp3::MlpGrid g{cos_device, log10e_device, hyp_device, skip_below_device, n_cos, n_bins};
     const int variant = h->forward_kernel == CP3H_MLP_LAYERED ? CP3H_MLP_FUSED4 : h->forward_kernel;
-    CP3H_HIP(cp3::launch_mlp_forward_fused(nullptr, n, y_device, h->wb, h->b, h->scaling, variant, &g, h->stream));
+    CP3H_HIP(cp3::launch_mlp_forward_fused(nullptr, n, y_device, h->wb, h->b, h->scaling, variant, &g, h->cu_count,
+                                           h->stream));
     return CP3H_OK;
 }
 

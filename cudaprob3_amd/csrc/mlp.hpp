@@ -82,7 +82,7 @@ struct MlpGrid {
 enum FusedVariant { kFusedRows128Waves4 = 1, kFusedRows128Waves8 = 2, kFusedRows64 = 3 };
 hipError_t launch_mlp_forward_fused(const float* x, long long n, float* y, const __bf16* const* w,
                                     const float* const* b, const MlpScaling& sc, int variant, const MlpGrid* grid,
-                                    hipStream_t s);
+                                    int cu_count, hipStream_t s);
 hipError_t launch_adam_vec(float* w, float* m1, float* m2, const float* g, int n, int n_valid, const AdamStep& a,
                            hipStream_t s);
 

@@ -31,6 +31,8 @@
 #include "mlp.hpp"
 #include "mlp_device.hpp"
 
+#include <algorithm>
+
 namespace cp3 {
 namespace {
 
@@ -160,13 +162,11 @@ __device__ __attribute__((always_inline)) inline void output_layer(const __bf16*
     }
 }
 
+// One tile: rows row0 .. row0 + kTileRows - 1 through all layers.
 template <int kTileRows, int kRowGroups, bool kGrid>
-__global__ __launch_bounds__(256 * kRowGroups) void mlp_forward_fused(FusedArgs a) {
-    __shared__ __bf16 act[kTileRows * kActStride];
-    __shared__ TileRows<kTileRows> tr;
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const long long row0 = (long long)blockIdx.x * kTileRows;
-
+__device__ __attribute__((always_inline)) inline void fused_tile(const FusedArgs& a, __bf16* act,
+                                                                 TileRows<kTileRows>& tr, long long row0, int wave,
+                                                                 int lane) {
     if (kGrid) {
         int needed = 0;
         if (threadIdx.x < kTileRows) {
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(256 * kRowGroups) void mlp_forward_fused(FusedArgs 
             tr.ib[threadIdx.x] = ib;
             needed = row < a.n && (!a.g.skip_below || ib >= a.g.skip_below[h * a.g.n_cos + ic]);
         }
-        if (!__syncthreads_or(needed)) return;      // the whole workgroup leaves together
+        if (!__syncthreads_or(needed)) return;      // the whole workgroup skips the tile together
     }
 
     // scaled inputs as two terms: hi in columns 0..8, lo in kMlpInLo.., zeros up to column 31
@@ -213,11 +213,33 @@ __global__ __launch_bounds__(256 * kRowGroups) void mlp_forward_fused(FusedArgs 
                                                wave, lane);
 }
 
+// Workgroup b takes tiles b, b + gridDim, b + 2 gridDim, ...: one tile each for an ordinary
+// launch (gridDim = tiles); in grid mode with skipped bins the launch is persistent (one
+// workgroup per resident slot, minus one so the stride is coprime with the tiles of a cosine
+// row) and unneeded tiles cost one barrier, with no dependence on how the hardware deals
+// workgroups to CUs (a per-row skip pattern in a one-tile-per-workgroup launch leaves CUs that
+// receive only skipped tiles idle while others queue).
+template <int kTileRows, int kRowGroups, bool kGrid, bool kPersistent>
+__global__ __launch_bounds__(256 * kRowGroups) void mlp_forward_fused(FusedArgs a) {
+    __shared__ __bf16 act[kTileRows * kActStride];
+    __shared__ TileRows<kTileRows> tr;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (!kPersistent) {
+        fused_tile<kTileRows, kRowGroups, kGrid>(a, act, tr, (long long)blockIdx.x * kTileRows, wave, lane);
+        return;
+    }
+    const long long tiles = (a.n + kTileRows - 1) / kTileRows;
+    for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        if (tile != (long long)blockIdx.x) __syncthreads();     // the previous tile's LDS reads are done
+        fused_tile<kTileRows, kRowGroups, kGrid>(a, act, tr, tile * kTileRows, wave, lane);
+    }
+}
+
 }  // namespace
 
 hipError_t launch_mlp_forward_fused(const float* x, long long n, float* y, const __bf16* const* w,
                                     const float* const* b, const MlpScaling& sc, int variant, const MlpGrid* grid,
-                                    hipStream_t s) {
+                                    int cu_count, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     FusedArgs a{x, n, y, {}, {}, sc, grid ? *grid : MlpGrid{}};
     for (int l = 0; l < kMlpLayers; l++) {
@@ -228,10 +250,24 @@ hipError_t launch_mlp_forward_fused(const float* x, long long n, float* y, const
     const int tile = variant == kFusedRows64 ? 64 : 128;
     const long long blocks = (n + tile - 1) / tile;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-#define CP3_FUSED(T, G)                                                                                      \
-    do {                                                                                                     \
-        if (grid) hipLaunchKernelGGL((mlp_forward_fused<T, G, true>), dim3(unsigned(blocks)), dim3(256 * G), 0, s, a); \
-        else hipLaunchKernelGGL((mlp_forward_fused<T, G, false>), dim3(unsigned(blocks)), dim3(256 * G), 0, s, a); \
+    // persistent launch (grid mode with skipped bins): resident workgroups of this kernel, minus one
+    auto resident = [&](const void* kernel, int threads) -> long long {
+        int per_cu = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        return std::max(1LL, (long long)per_cu * cu_count - 1);
+    };
+#define CP3_FUSED(T, G)                                                                                           \
+    do {                                                                                                          \
+        if (grid && grid->skip_below) {                                                                           \
+            const void* k = reinterpret_cast<const void*>(&mlp_forward_fused<T, G, true, true>);                 \
+            const long long launch = std::min(blocks, resident(k, 256 * G));                                     \
+            hipLaunchKernelGGL((mlp_forward_fused<T, G, true, true>), dim3(unsigned(launch)), dim3(256 * G), 0, s, a); \
+        } else if (grid) {                                                                                        \
+            hipLaunchKernelGGL((mlp_forward_fused<T, G, true, false>), dim3(unsigned(blocks)), dim3(256 * G), 0, s, a); \
+        } else {                                                                                                  \
+            hipLaunchKernelGGL((mlp_forward_fused<T, G, false, false>), dim3(unsigned(blocks)), dim3(256 * G), 0, s, a); \
+        }                                                                                                         \
     } while (0)
     switch (variant) {
         case kFusedRows128Waves4: CP3_FUSED(128, 1); break;

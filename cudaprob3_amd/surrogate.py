@@ -332,6 +332,7 @@ class GridEvaluator:
                  slots: int = 8, device: int = 0):
         torch = net.torch
         self.torch, self.net, self.slots = torch, net, slots
+        net.set_forward_kernel("fused8")    # fastest persistent form for grids with skipped bins
         self.cos = np.asarray(cos, np.float64)
         self.centers = np.asarray(centers, np.float64)
         n_c, n_b = len(self.cos), len(self.centers)
