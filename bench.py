@@ -370,10 +370,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="jitter12", choices=["jitter12", "scan1024", "mcmc8192"],
-                    help="jitter12: the 2048^2 grid, four variants per step; scan1024: 1024 parameter sets "
-                         "with uneven grids and Earth models in one persistent launch per step; mcmc8192: "
-                         "Markov-chain steps over an 8192^2 grid on a 100-shell Earth")
+    ap.add_argument("--workload", default="jitter12", choices=["jitter12", "scan1024", "mcmc8192", "surrogate2048"],
+                    help="jitter12: the 2048^2 grid, four variants per step (plus the surrogate comparison); "
+                         "scan1024: 1024 parameter sets with uneven grids and Earth models in one persistent launch "
+                         "per step; mcmc8192: Markov-chain steps over an 8192^2 grid on a 100-shell Earth; "
+                         "surrogate2048: the surrogate comparison alone")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64", "fp64_plain"])
     ap.add_argument("--n", type=int, default=2048, help="grid edge (cosines = energies = n)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -416,6 +417,20 @@ def main():
         return
     if args.workload == "mcmc8192":
         print(json.dumps(run_mcmc(args, sync)), flush=True)
+        return
+    if args.workload == "surrogate2048":
+        t0 = time.perf_counter()
+        sur = surrogate_comparison(args, sync)
+        bins = args.surrogate_hypotheses * args.n * args.n
+        print(json.dumps({
+            "metric": "bins/s: surrogate + exact fallback at max|dP|<=1e-3 (binned probabilities, 2048^2 grid)",
+            "value": sur["bins_per_s"], "unit": "bins/s", "n_gpus": 1, "steps": args.surrogate_hypotheses,
+            "warmup": 8, "ms_per_step": bins / sur["bins_per_s"] / args.surrogate_hypotheses * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "bf16 MFMA (fp32 accumulation) + f32/f64 exact fallback", "data": "synthetic",
+            "config": {"workload": "surrogate2048: " + sur["workload"]},
+            "roofline": sur["network_roofline"], "surrogate": sur,
+            "wall_s": time.perf_counter() - t0}), flush=True)
         return
 
     grids = W.jitter12_variants(args.n)
