@@ -219,7 +219,48 @@ def run_mcmc(args, sync):
                      "kernel_ms": k_ms, "all_kernels_ms": total_ms, "algorithmic_flops_per_launch": flops},
     }
     chain.close()
+    if args.binned_chain_steps > 0:
+        result["binned_chains"] = binned_chain_comparison(args, sync, grid)
     return result
+
+
+def binned_chain_comparison(args, sync, grid):
+    """The same chain on bin-averaged probabilities (the grid's energies as bin centres): the
+    exact kernel alone (16 Gauss-Legendre nodes per bin) against the surrogate network trained
+    for the 100-shell Earth plus its exact fallback."""
+    import torch
+    from cudaprob3_amd import train_surrogate as T
+    from cudaprob3_amd import workloads as W
+    from cudaprob3_amd.mcmc import BinnedChain
+
+    path = T.path_for("fine100")
+    if not os.path.exists(path):
+        return {"skipped": f"{os.path.basename(path)} not built (python -m cudaprob3_amd.train_surrogate --earth fine100)"}
+    net, emap, meta = T.load(path, max_batch=8192)
+    out, truth = {}, {}
+    for method in ("exact", "surrogate"):
+        ch = BinnedChain(grid, W.NUFIT, method, net=net, emap=emap, earth="fine100", sigma_p=0.1, seed=5)
+        truth[method] = ch.data.clone()
+        ch.run(2)
+        ch.stats = type(ch.stats)()
+        sync()
+        t0 = time.perf_counter()
+        st = ch.run(args.binned_chain_steps)
+        sync()
+        dt = time.perf_counter() - t0
+        out[method] = {"steps": st.steps, "p50_ms": st.percentile(50), "p99_ms": st.percentile(99),
+                       "bins_per_s": st.steps * ch.n_bins / dt, "acceptance": st.accepted / max(st.steps, 1)}
+        if method == "surrogate":
+            out[method]["exact_fraction"] = ch.exact_fraction
+        log(f"[bench] binned chain ({method}): p50 {out[method]['p50_ms']:.2f} ms, p99 {out[method]['p99_ms']:.2f} ms, "
+            f"{out[method]['bins_per_s']:.4e} bins/s")
+        ch.close()
+    out["max_abs_dP_surrogate_vs_exact_at_truth"] = float((truth["surrogate"] - truth["exact"]).abs().max())
+    out["network"] = meta
+    log(f"[bench] binned chains: surrogate+fallback vs exact at the true parameters max|dP| "
+        f"{out['max_abs_dP_surrogate_vs_exact_at_truth']:.3e}")
+    net.close()
+    return out
 
 
 def surrogate_comparison(args, sync):
@@ -382,6 +423,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the other precisions' throughput")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate-vs-exact comparison")
     ap.add_argument("--surrogate-hypotheses", type=int, default=256)
+    ap.add_argument("--binned-chain-steps", type=int, default=20,
+                    help="mcmc8192: steps of the binned exact and surrogate chains (0: skip)")
     ap.add_argument("--surrogate-validate", type=int, default=64,
                     help="hypotheses whose every bin is checked against the exact binned kernel")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,

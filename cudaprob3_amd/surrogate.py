@@ -33,6 +33,9 @@ FEATURE_LO = np.array([-1.0, LOG10E_RANGE[0], 0.55, 0.14, 0.65, 0.0, 6.8e-5, -2.
 FEATURE_HI = np.array([1.0, LOG10E_RANGE[1], 0.62, 0.16, 0.92, 2 * np.pi, 8.0e-5, 2.7e-3, 1.0], np.float32)
 
 GL_NODES, GL_WEIGHTS = np.polynomial.legendre.leggauss(16)
+
+# Earth models a network can be trained for (it is valid for that model only)
+EARTHS = {"jitter12": W.jitter12_model, "fine100": W.fine100_model}
 BIN_HALF_WIDTH = 0.025
 
 
@@ -184,11 +187,11 @@ class BinnedSets:
     specs: list
 
 
-def exact_binned(specs_bins, device=0):
-    """specs_bins: list of (cosines, bin centres, params, antinu) on the jittered 12-shell
-    Earth.  Returns per-set device tensors [9][n_cos][n_bins] of GL bin averages."""
+def exact_binned(specs_bins, device=0, earth: str = "jitter12"):
+    """specs_bins: list of (cosines, bin centres, params, antinu) on the named Earth model.
+    Returns per-set device tensors [9][n_cos][n_bins] of GL bin averages."""
     import torch
-    radii, rhos = W.jitter12_model()
+    radii, rhos = EARTHS[earth]()
     specs, weights = [], None
     for cos, centers, p, nu in specs_bins:
         E, weights = gl_energies(centers)
@@ -214,7 +217,7 @@ def random_hypothesis(rng):
 
 
 def make_dataset(n_sets: int, n_cos: int, n_bins: int, seed: int, device=0, chunk: int = 256,
-                 log10e_range=LOG10E_RANGE) -> BinnedSets:
+                 log10e_range=LOG10E_RANGE, earth: str = "jitter12") -> BinnedSets:
     """Random hypotheses (parameter ranges of the scan workload, either ordering, nu or
     nubar), each on n_cos random cosines x n_bins random log-uniform bin centres in
     10**log10e_range GeV (default [0.05, 1000]); exact GL-binned targets."""
@@ -229,7 +232,7 @@ def make_dataset(n_sets: int, n_cos: int, n_bins: int, seed: int, device=0, chun
             cos = np.sort(rng.uniform(-1.0, 1.0, n_cos))
             l10 = np.sort(rng.uniform(log10e_range[0], log10e_range[1], n_bins))
             batch.append((cos, 10.0 ** l10, p, nu))
-        for (cos, centers, p, nu), P in zip(batch, exact_binned(batch, device)):
+        for (cos, centers, p, nu), P in zip(batch, exact_binned(batch, device, earth)):
             Xs.append(features(torch, cos, np.log10(centers), p, nu, dev))
             Ts.append(P.permute(1, 2, 0).reshape(-1, 9))
             specs.append((p, nu))
@@ -329,7 +332,7 @@ class GridEvaluator:
     """
 
     def __init__(self, net: SurrogateNet, emap: ErrorMap, cos, centers, tol: float = 1e-3, safety: float = 1.5,
-                 slots: int = 8, device: int = 0):
+                 slots: int = 8, device: int = 0, earth: str = "jitter12", classes=range(N_CLASSES)):
         torch = net.torch
         self.torch, self.net, self.slots = torch, net, slots
         net.set_forward_kernel("fused8")    # fastest persistent form for grids with skipped bins
@@ -343,10 +346,15 @@ class GridEvaluator:
         self.cos_t = torch.as_tensor(self.cos.astype(np.float32), device=dev)
         self.l10_t = torch.as_tensor(l10.astype(np.float32), device=dev)
         self.out = torch.empty((slots, 9, n_c, n_b), dtype=torch.float32, device=dev)
-        radii, rhos = W.jitter12_model()
+        radii, rhos = EARTHS[earth]()
         E_nodes, self.weights = gl_energies(self.centers)
         self.skip, self.skip_t, self.batches = [], [], []
         for k in range(N_CLASSES):
+            if k not in classes:                 # hypotheses of this class are not expected
+                self.skip.append(None)
+                self.skip_t.append(None)
+                self.batches.append(None)
+                continue
             mask = emap.fallback(k, self.cos, l10, tol, safety)
             last = np.where(mask.any(axis=1), n_b - np.argmax(mask[:, ::-1], axis=1), 0)   # prefix lengths
             self.skip.append(last.astype(np.int32))
@@ -366,7 +374,7 @@ class GridEvaluator:
         torch.cuda.synchronize(dev)
 
     def fallback_fraction(self, k: int) -> float:
-        return float(self.skip[k].sum()) / self.plane
+        return float(self.skip[k].sum()) / self.plane if self.skip[k] is not None else float("nan")
 
     def run(self, params, antinu, consume=None) -> dict:
         """Evaluates every hypothesis; returns counts (bins, exact bins) per class."""
@@ -376,6 +384,8 @@ class GridEvaluator:
         i = 0
         while i < len(order):
             k = hypothesis_class(params[order[i]], antinu[order[i]])
+            if self.skip[k] is None:
+                raise ValueError(f"this evaluator was not built for hypothesis class {k}")
             chunk = []
             while i < len(order) and len(chunk) < self.slots and \
                     hypothesis_class(params[order[i]], antinu[order[i]]) == k:
@@ -410,7 +420,7 @@ class ExactBinnedGrid:
     """The exact mixed-precision kernel alone on the same grid: every bin from 16 GL nodes
     (batched, binned mode), `slots` hypotheses per launch, into ``self.out``."""
 
-    def __init__(self, torch, cos, centers, slots: int = 8, device: int = 0):
+    def __init__(self, torch, cos, centers, slots: int = 8, device: int = 0, earth: str = "jitter12"):
         self.torch, self.slots = torch, slots
         self.cos = np.asarray(cos, np.float64)
         self.centers = np.asarray(centers, np.float64)
@@ -418,7 +428,7 @@ class ExactBinnedGrid:
         self.plane = n_c * n_b
         self.dev = torch.device(f"cuda:{device}")
         self.out = torch.empty((slots, 9, n_c, n_b), dtype=torch.float32, device=self.dev)
-        radii, rhos = W.jitter12_model()
+        radii, rhos = EARTHS[earth]()
         E_nodes, w = gl_energies(self.centers)
         specs = [SetSpec(self.cos, E_nodes, radii, rhos, (0.59, 0.15, 0.78, 0.0, 7.4e-5, 2.5e-3), 0)
                  for _ in range(slots)]

@@ -1,13 +1,14 @@
 """Trains the surrogate network on exact bin-averaged probabilities and writes its parameters
-and error map (cudaprob3_amd/data/surrogate_jitter12.npz by default).
+and error map (cudaprob3_amd/data/surrogate_<earth>.npz).
 
-    python -m cudaprob3_amd.train_surrogate [--samples 50331648] [--epochs 40] [--min-energy 2]
+    python -m cudaprob3_amd.train_surrogate [--earth jitter12|fine100] [--samples 50331648]
+                                            [--epochs 50] [--min-energy 2]
 
 Training data: `samples` points (cosZ, energy-bin centre, hypothesis) in sets of 64 random
 cosines x 64 log-uniform bin centres per random hypothesis (parameter ranges of the scan
 workload, either ordering, nu or nubar), targets from the exact mixed-precision kernel
-averaged over +-2.5 % bins by 16-point Gauss-Legendre quadrature, on the jittered 12-shell
-Earth.  Bins below --min-energy GeV are left out of training (they always fall back to the
+averaged over +-2.5 % bins by 16-point Gauss-Legendre quadrature, on one Earth model (the
+jittered 12-shell profile, or the 100-shell profile of the Markov-chain workload).  Bins below --min-energy GeV are left out of training (they always fall back to the
 exact kernel).  The error map comes from 1024 further hypotheses over the full energy range.
 Runs on one GPU in a few minutes.
 """
@@ -23,7 +24,12 @@ import numpy as np
 
 from . import surrogate as S
 
-DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_jitter12.npz")
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+DEFAULT_PATH = os.path.join(DATA_DIR, "surrogate_jitter12.npz")
+
+
+def path_for(earth: str) -> str:
+    return os.path.join(DATA_DIR, f"surrogate_{earth}.npz")
 
 
 def save(path, net, emap: S.ErrorMap, meta: dict):
@@ -45,28 +51,32 @@ def load(path: str = DEFAULT_PATH, max_batch: int = 8192, device: int = 0):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--samples", type=int, default=50_331_648)
-    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--epochs", type=int, default=50)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--min-energy", type=float, default=2.0)
     ap.add_argument("--seed", type=int, default=101)
-    ap.add_argument("--out", default=DEFAULT_PATH)
+    ap.add_argument("--earth", default="jitter12", choices=sorted(S.EARTHS))
+    ap.add_argument("--out", default=None, help="default: cudaprob3_amd/data/surrogate_<earth>.npz")
     a = ap.parse_args(argv)
+    out = a.out or path_for(a.earth)
     t0 = time.time()
     lo = math.log10(max(a.min_energy, 10 ** S.LOG10E_RANGE[0]))
     n_sets = a.samples // 4096
-    data = S.make_dataset(n_sets=n_sets, n_cos=64, n_bins=64, seed=a.seed, log10e_range=(lo, S.LOG10E_RANGE[1]))
+    data = S.make_dataset(n_sets=n_sets, n_cos=64, n_bins=64, seed=a.seed, log10e_range=(lo, S.LOG10E_RANGE[1]),
+                          earth=a.earth)
     print(f"training data: {data.X.shape[0]} samples from {n_sets} hypotheses ({time.time() - t0:.1f} s)", flush=True)
     net = S.SurrogateNet(max_batch=65536, training=True, seed=3)
     t1 = time.time()
     hist = S.train(net, data, epochs=a.epochs, batch=a.batch, lr_max=a.lr, seed=a.seed,
                    log=lambda s: print(f"[{time.time() - t1:.0f} s] {s}", flush=True))
     del data
-    held = S.make_dataset(n_sets=1024, n_cos=64, n_bins=64, seed=a.seed + 1)
+    held = S.make_dataset(n_sets=1024, n_cos=64, n_bins=64, seed=a.seed + 1, earth=a.earth)
     emap = S.build_error_map(net, held)
-    meta = dict(samples=n_sets * 4096, epochs=a.epochs, batch=a.batch, lr=a.lr, min_energy_gev=10 ** lo, seed=a.seed,
+    meta = dict(earth=a.earth, samples=n_sets * 4096, epochs=a.epochs, batch=a.batch, lr=a.lr,
+                min_energy_gev=10 ** lo, seed=a.seed,
                 final_loss=hist[-1] if hist else None, train_seconds=time.time() - t1, held_out_samples=held.X.shape[0])
-    save(a.out, net, emap, meta)
+    save(out, net, emap, meta)
     print(json.dumps(meta), flush=True)
 
 
