@@ -87,6 +87,9 @@ SIGNATURES = {
     "cp3h_mlp_set_forward_kernel": (ctypes.c_int, [_handle, ctypes.c_int]),
     "cp3h_mlp_forward_grid": (ctypes.c_int, [_handle, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "cp3h_mlp_forward_grid_tiles": (ctypes.c_int, [_handle, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                                   ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                   ctypes.c_longlong, ctypes.c_void_p]),
     "cp3h_mlp_gradients": (ctypes.c_int, [_handle, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_float_p,
                                           ctypes.c_size_t, _c_float_p]),
     "cp3h_mlp_train_step": (ctypes.c_int, [_handle, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float,

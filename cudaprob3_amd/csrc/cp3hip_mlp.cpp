@@ -322,7 +322,7 @@ int cp3h_mlp_forward(cp3h_mlp* h, const float* x_device, long long n, float* y_d
     CP3H_HIP(hipSetDevice(h->device));
     if (h->forward_kernel != CP3H_MLP_LAYERED) {
         CP3H_HIP(cp3::launch_mlp_forward_fused(x_device, n, y_device, h->wb, h->b, h->scaling, h->forward_kernel,
-                                               nullptr, h->cu_count, h->stream));
+                                               nullptr, h->cu_count, 0, h->stream));
         return CP3H_OK;
     }
     for (long long r0 = 0; r0 < n; r0 += h->max_batch) {
@@ -341,10 +341,27 @@ int cp3h_mlp_forward_grid(cp3h_mlp* h, const float* cos_device, int n_cos, const
     if (!cos_device || !log10e_device || !hyp_device || !y_device)
         return fail(CP3H_ERR_ARGUMENT, "null grid arrays");
     CP3H_HIP(hipSetDevice(h->device));
-    const cp3::MlpGrid g{cos_device, log10e_device, hyp_device, skip_below_device, n_cos, n_bins};
+    const cp3::MlpGrid g{cos_device, log10e_device, hyp_device, skip_below_device, nullptr, n_cos, n_bins};
     const int variant = h->forward_kernel == CP3H_MLP_LAYERED ? CP3H_MLP_FUSED4 : h->forward_kernel;
-    CP3H_HIP(cp3::launch_mlp_forward_fused(nullptr, n, y_device, h->wb, h->b, h->scaling, variant, &g, h->cu_count,
+    CP3H_HIP(cp3::launch_mlp_forward_fused(nullptr, n, y_device, h->wb, h->b, h->scaling, variant, &g, h->cu_count, 0,
                                            h->stream));
+    return CP3H_OK;
+}
+
+int cp3h_mlp_forward_grid_tiles(cp3h_mlp* h, const float* cos_device, int n_cos, const float* log10e_device,
+                                int n_bins, const float* hyp_device, int n_hyp, const int* skip_below_device,
+                                const long long* tiles_device, long long n_tiles, float* y_device) {
+    if (int rc = check(h)) return rc;
+    if (n_cos <= 0 || n_bins <= 0 || n_hyp < 0 || n_tiles < 0) return fail(CP3H_ERR_ARGUMENT, "bad grid size");
+    if (n_tiles == 0 || n_hyp == 0) return CP3H_OK;
+    if (!cos_device || !log10e_device || !hyp_device || !y_device || !tiles_device)
+        return fail(CP3H_ERR_ARGUMENT, "null grid arrays");
+    CP3H_HIP(hipSetDevice(h->device));
+    const long long n = (long long)n_cos * n_bins * n_hyp;
+    const cp3::MlpGrid g{cos_device, log10e_device, hyp_device, skip_below_device, tiles_device, n_cos, n_bins};
+    const int variant = h->forward_kernel == CP3H_MLP_FUSED8 ? CP3H_MLP_FUSED8 : CP3H_MLP_FUSED4;
+    CP3H_HIP(cp3::launch_mlp_forward_fused(nullptr, n, y_device, h->wb, h->b, h->scaling, variant, &g, h->cu_count,
+                                           n_tiles, h->stream));
     return CP3H_OK;
 }
 

@@ -74,6 +74,7 @@ struct MlpGrid {
     const float* log10e;      // [n_bins] log10 of the bin centres / GeV
     const float* hyp;         // [n_hyp][7]: theta12, theta13, theta23, deltaCP, dm21^2, dm32^2, nubar flag
     const int* skip_below;    // optional [n_hyp][n_cos]: rows with ib below it may be left unwritten
+    const long long* tiles;   // optional: the 128-row tiles to compute (tile t = rows 128 t ..), one per workgroup
     int n_cos, n_bins;
 };
 
@@ -82,7 +83,7 @@ struct MlpGrid {
 enum FusedVariant { kFusedRows128Waves4 = 1, kFusedRows128Waves8 = 2, kFusedRows64 = 3 };
 hipError_t launch_mlp_forward_fused(const float* x, long long n, float* y, const __bf16* const* w,
                                     const float* const* b, const MlpScaling& sc, int variant, const MlpGrid* grid,
-                                    int cu_count, hipStream_t s);
+                                    int cu_count, long long list_len, hipStream_t s);
 hipError_t launch_adam_vec(float* w, float* m1, float* m2, const float* g, int n, int n_valid, const AdamStep& a,
                            hipStream_t s);
 

@@ -15,18 +15,20 @@
 // registers until every wavefront has finished reading the layer's inputs (workgroup barrier),
 // then overwrite them in place.
 //
-// Variants: 128-row tiles with 4 wavefronts (each over all rows) or 8 (row halves x 4 feature
-// slices), one workgroup per CU (133 KB of LDS); 64-row tiles with 4 wavefronts and 66.5 KB, two
-// workgroups per CU whose layers drift out of phase, so one's GELU epilogue (vector ALU) runs
-// beside the other's MFMAs.
+// Variants: 128-row tiles with 4 wavefronts (each over all rows; the default, fastest: 7.1e8
+// rows/s) or 8 (row halves x 4 feature slices), one workgroup per CU (133 KB of LDS); 64-row
+// tiles with 4 wavefronts and 66.5 KB (two workgroups per CU where registers allow).  Within a
+// wavefront the GELU epilogue of a 16-row block follows its MFMAs; the compiler does not
+// interleave the two.
 //
 // Grid mode (kGrid): the rows are the points (hypothesis h, cosine ic, energy bin ib) of
 // n_hyp hypotheses on one cosZ x bin grid, h-major then ic then ib; the kernel forms each
 // row's inputs (cos[ic], log10e[ib], the hypothesis' 6 parameters and nu / nubar flag) itself
 // and stores the outputs in the exact kernels' SoA layout, y[h][t][ic][ib].  With skip_below,
 // rows with ib < skip_below[h * n_cos + ic] (bins that the exact kernel provides) are not
-// stored, so the two kernels can fill one array concurrently, and a tile made only of such
-// rows exits before any work.
+// stored, so the two kernels fill disjoint parts of one array, and a tile made only of such
+// rows is skipped after one barrier (persistent launch, see mlp_forward_fused) -- or, with a
+// list of the tiles to compute, only those tiles are launched.
 #include "common.hpp"
 #include "mlp.hpp"
 #include "mlp_device.hpp"
@@ -225,7 +227,8 @@ __global__ __launch_bounds__(256 * kRowGroups) void mlp_forward_fused(FusedArgs 
     __shared__ TileRows<kTileRows> tr;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     if (!kPersistent) {
-        fused_tile<kTileRows, kRowGroups, kGrid>(a, act, tr, (long long)blockIdx.x * kTileRows, wave, lane);
+        const long long tile = kGrid && a.g.tiles ? a.g.tiles[blockIdx.x] : (long long)blockIdx.x;
+        fused_tile<kTileRows, kRowGroups, kGrid>(a, act, tr, tile * kTileRows, wave, lane);
         return;
     }
     const long long tiles = (a.n + kTileRows - 1) / kTileRows;
@@ -239,7 +242,7 @@ __global__ __launch_bounds__(256 * kRowGroups) void mlp_forward_fused(FusedArgs 
 
 hipError_t launch_mlp_forward_fused(const float* x, long long n, float* y, const __bf16* const* w,
                                     const float* const* b, const MlpScaling& sc, int variant, const MlpGrid* grid,
-                                    int cu_count, hipStream_t s) {
+                                    int cu_count, long long list_len, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     FusedArgs a{x, n, y, {}, {}, sc, grid ? *grid : MlpGrid{}};
     for (int l = 0; l < kMlpLayers; l++) {
@@ -247,6 +250,17 @@ hipError_t launch_mlp_forward_fused(const float* x, long long n, float* y, const
         a.b[l] = b[l];
     }
     if (grid && (grid->n_cos <= 0 || grid->n_bins <= 0)) return hipErrorInvalidValue;
+    if (grid && grid->tiles) {
+        // listed 128-row tiles, one workgroup each (no skipped workgroups)
+        if (variant == kFusedRows64) variant = kFusedRows128Waves4;
+        if (list_len <= 0) return list_len == 0 ? hipSuccess : hipErrorInvalidValue;
+        if (list_len > 0x7fffffffLL) return hipErrorInvalidValue;
+        if (variant == kFusedRows128Waves8)
+            hipLaunchKernelGGL((mlp_forward_fused<128, 2, true, false>), dim3(unsigned(list_len)), dim3(512), 0, s, a);
+        else
+            hipLaunchKernelGGL((mlp_forward_fused<128, 1, true, false>), dim3(unsigned(list_len)), dim3(256), 0, s, a);
+        return hipGetLastError();
+    }
     const int tile = variant == kFusedRows64 ? 64 : 128;
     const long long blocks = (n + tile - 1) / tile;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;

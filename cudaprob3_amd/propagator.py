@@ -196,8 +196,9 @@ class HipPropagatorSingle(Propagator):
 
     ``HipPropagatorSingle(id, n_cosines, n_energies)`` or ``(n_cosines, n_energies)``
     (cudapropagator.cuh:50, 86).  ``precision``: ``"fp64_plain"`` (the reference's per-pair
-    FP64 algorithm, default for float64), ``"fp64"`` (hoisted FP64 spectrum), ``"mixed"``
-    (FP32 bulk with FP64 eigenvalues and phases, float results; default for float32).
+    FP64 algorithm), ``"fp64"`` (hoisted FP64 spectrum, same parity bar; default for float64,
+    as in the C++ class), ``"mixed"`` (FP32 bulk with FP64 eigenvalues and phases, float
+    results; default for float32).
     """
 
     def __init__(self, *args, dtype=np.float64, precision: str | None = None):
@@ -209,7 +210,7 @@ class HipPropagatorSingle(Propagator):
             raise TypeError("HipPropagatorSingle(id, n_cosines, n_energies) or (n_cosines, n_energies)")
         super().__init__(n_c, n_e, dtype)
         if precision is None:
-            precision = "fp64_plain" if self.dtype == np.float64 else "mixed"
+            precision = "fp64" if self.dtype == np.float64 else "mixed"
         if precision not in _lib.PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(_lib.PRECISIONS)}")
         self.precision = precision

@@ -132,6 +132,19 @@ class SurrogateNet:
             ctypes.c_void_p(skip_t.data_ptr() if skip_t is not None else None), ctypes.c_void_p(out_t.data_ptr())))
         self._before_torch()
 
+    def forward_grid_tiles(self, cos_t, log10e_t, hyp_t, skip_t, tiles_t, out_t):
+        """Grid-mode inference of the listed 128-row tiles only (tiles_t: device int64 tile
+        indices of the h-major row order); other points and those below skip_t stay unwritten."""
+        n_hyp = hyp_t.shape[0]
+        assert out_t.numel() >= n_hyp * 9 * cos_t.numel() * log10e_t.numel()
+        self._after_torch()
+        _lib.check(self._lib.cp3h_mlp_forward_grid_tiles(
+            self._h, ctypes.c_void_p(cos_t.data_ptr()), cos_t.numel(), ctypes.c_void_p(log10e_t.data_ptr()),
+            log10e_t.numel(), ctypes.c_void_p(hyp_t.data_ptr()), n_hyp,
+            ctypes.c_void_p(skip_t.data_ptr() if skip_t is not None else None), ctypes.c_void_p(tiles_t.data_ptr()),
+            tiles_t.numel(), ctypes.c_void_p(out_t.data_ptr())))
+        self._before_torch()
+
     def sync(self):
         _lib.check(self._lib.cp3h_mlp_sync(self._h))
 
@@ -335,7 +348,7 @@ class GridEvaluator:
                  slots: int = 8, device: int = 0, earth: str = "jitter12", classes=range(N_CLASSES)):
         torch = net.torch
         self.torch, self.net, self.slots = torch, net, slots
-        net.set_forward_kernel("fused8")    # fastest persistent form for grids with skipped bins
+        net.set_forward_kernel("fused4")    # listed tiles run the one-tile-per-workgroup kernel
         self.cos = np.asarray(cos, np.float64)
         self.centers = np.asarray(centers, np.float64)
         n_c, n_b = len(self.cos), len(self.centers)
@@ -348,17 +361,23 @@ class GridEvaluator:
         self.out = torch.empty((slots, 9, n_c, n_b), dtype=torch.float32, device=dev)
         radii, rhos = EARTHS[earth]()
         E_nodes, self.weights = gl_energies(self.centers)
-        self.skip, self.skip_t, self.batches = [], [], []
+        self.skip, self.skip_t, self.tiles_t, self.batches = [], [], [], []
         for k in range(N_CLASSES):
             if k not in classes:                 # hypotheses of this class are not expected
                 self.skip.append(None)
                 self.skip_t.append(None)
+                self.tiles_t.append(None)
                 self.batches.append(None)
                 continue
             mask = emap.fallback(k, self.cos, l10, tol, safety)
             last = np.where(mask.any(axis=1), n_b - np.argmax(mask[:, ::-1], axis=1), 0)   # prefix lengths
             self.skip.append(last.astype(np.int32))
             self.skip_t.append(torch.as_tensor(np.tile(last.astype(np.int32), (slots, 1)), device=dev))
+            # 128-row tiles (h-major row order over the slots) holding at least one served bin
+            served = np.tile((np.arange(n_b)[None, :] >= last[:, None]).reshape(-1), slots)
+            pad = (-served.size) % 128
+            tiles = np.flatnonzero(np.concatenate([served, np.zeros(pad, bool)]).reshape(-1, 128).any(axis=1))
+            self.tiles_t.append(torch.as_tensor(tiles.astype(np.int64), device=dev))
             kmax = int(last.max())
             if kmax == 0:
                 self.batches.append(None)
@@ -395,7 +414,7 @@ class GridEvaluator:
             rows += [rows[-1]] * (self.slots - len(chunk))        # idle slots repeat the last one
             hyp = torch.tensor(rows, dtype=torch.float32, device=self.dev)
             torch.cuda.synchronize(self.dev)          # previous chunk consumed; hyp uploaded
-            net.forward_grid(self.cos_t, self.l10_t, hyp, self.skip_t[k], self.out)
+            net.forward_grid_tiles(self.cos_t, self.l10_t, hyp, self.skip_t[k], self.tiles_t[k], self.out)
             bp = self.batches[k]
             if bp is not None:
                 bp.set_params_many(np.array([r[:6] for r in rows], np.float64), np.full(self.slots, k // 2, np.int32))

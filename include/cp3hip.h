@@ -239,6 +239,13 @@ int cp3h_mlp_set_forward_kernel(cp3h_mlp* h, int kernel);
    possibly at the same time). */
 int cp3h_mlp_forward_grid(cp3h_mlp* h, const float* cos_device, int n_cos, const float* log10e_device, int n_bins,
                           const float* hyp_device, int n_hyp, const int* skip_below_device, float* y_device);
+/* The same over listed tiles only: tiles_device[0 .. n_tiles) are indices t of 128-row tiles of
+   the h-major row order (rows 128 t .. 128 t + 127, each < n_hyp * n_cos * n_bins, no repeats);
+   points of other tiles, and points below skip_below (may be NULL), are left unwritten.  Uses
+   FUSED8 when that kernel is selected, else FUSED4. */
+int cp3h_mlp_forward_grid_tiles(cp3h_mlp* h, const float* cos_device, int n_cos, const float* log10e_device,
+                                int n_bins, const float* hyp_device, int n_hyp, const int* skip_below_device,
+                                const long long* tiles_device, long long n_tiles, float* y_device);
 /* gradients of the mean squared error over the 9 outputs of one batch (M a multiple of 256,
    <= max_batch), copied to host memory in the parameter layout, and the loss */
 int cp3h_mlp_gradients(cp3h_mlp* h, const float* x_device, const float* t_device, int M, float* grads, size_t n,

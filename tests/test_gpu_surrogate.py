@@ -159,6 +159,22 @@ def test_forward_grid_matches_row_mode(torch):
         keep9 = np.broadcast_to(keep[:, None], got.shape)
         np.testing.assert_allclose(got[keep9], ref[keep9], rtol=0, atol=1e-6)
         assert np.all(got[~keep9] == -3.0)
+    # listed tiles only (128-row tiles of the h-major row order)
+    n_rows = n_hyp * n_c * n_b
+    tiles = np.array(sorted(np.random.default_rng(2).choice((n_rows + 127) // 128, 9, replace=False)), np.int64)
+    in_tile = np.zeros(n_rows, bool)
+    for t in tiles:
+        in_tile[t * 128:(t + 1) * 128] = True
+    in_tile = in_tile.reshape(n_hyp, n_c, n_b)
+    for kernel in ("fused4", "fused8"):
+        net.set_forward_kernel(kernel)
+        out = torch.full((n_hyp, 9, n_c, n_b), -3.0, device="cuda")
+        net.forward_grid_tiles(dev(cos), dev(l10), dev(hyp), dev(skip), dev(tiles), out)
+        got = out.cpu().numpy()
+        keep = in_tile & (np.arange(n_b)[None, None, :] >= skip[:, :, None])
+        keep9 = np.broadcast_to(keep[:, None], got.shape)
+        np.testing.assert_allclose(got[keep9], ref[keep9], rtol=0, atol=1e-6)
+        assert np.all(got[~keep9] == -3.0)
     net.close()
 
 
