@@ -1,5 +1,5 @@
 """Learned surrogate of energy-bin-averaged oscillation probabilities, with the exact kernel as
-its data source, its fallback and the judge of its error.
+its data source, its fallback and the measure of its error.
 
 * Network: 9 -> 256 x 5 (GELU, tanh form) -> 9, BF16 weights, FP32 accumulation, activations
   carried as two bf16 terms; trained and evaluated by the library's matrix-core kernels
