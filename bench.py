@@ -64,12 +64,16 @@ def kernel_ms(props, grids, reps):
     """Average GPU duration of the per-pair kernel and of all kernels, per launch, with one
     launch in flight at a time."""
     pairs, total = [], []
+    for p in props:
+        p.set_timing(True)
     for _ in range(reps):
         for p, g in zip(props, grids):
             p.calculateProbabilities(g.antinu)
             t, k = p.last_compute_times()
             total.append(t)
             pairs.append(k)
+    for p in props:
+        p.set_timing(False)
     return float(np.mean(pairs)), float(np.mean(total))
 
 
@@ -197,6 +201,7 @@ def run_mcmc(args, sync):
     dt = time.perf_counter() - t0
     value = args.steps * chain.n_pairs / dt
     p50, p99 = st.percentile(50), st.percentile(99)
+    chain.prop.set_timing(True)
     chain.prop.calculateProbabilities(grid.antinu)
     total_ms, k_ms = chain.prop.last_compute_times()
     ml = M.maxlayers_of(grid.cosines, grid.radii, grid.rhos)

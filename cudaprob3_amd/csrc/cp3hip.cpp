@@ -43,6 +43,7 @@ struct cp3h_propagator {
     cp3::OscParams params{};
     bool has_energies = false, has_cosines = false, has_density = false, has_params = false;
     bool has_timing = false;
+    bool timing = false;               // record HIP events around the kernels (cp3h_set_timing)
 
     std::size_t n_results() const { return std::size_t(9) * n_c * n_e; }
 };
@@ -280,29 +281,29 @@ int cp3h_compute_async(cp3h_propagator* h, int antineutrino, double production_h
     h->params.antinu = antineutrino;
     const cp3::GridDesc g{h->d_cos, h->d_energies, h->d_maxlayers, h->d_radii, h->d_rhos,
                           h->n_c, h->n_e, h->rows, production_height_cm};
-    CP3H_HIP(hipEventRecord(h->ev_start, h->stream));
+    if (h->timing) CP3H_HIP(hipEventRecord(h->ev_start, h->stream));
     if (h->n_c == 0 || h->n_e == 0) {          // empty grid: nothing to launch
-        CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
+        if (h->timing) CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
     } else if (h->precision == CP3H_FP64_PLAIN) {
-        CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
+        if (h->timing) CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
         CP3H_HIP(cp3::launch_plain_f64(g, h->params, static_cast<double*>(h->d_out), h->stream));
     } else {
         if (h->precision == CP3H_MIXED) {
             CP3H_HIP(cp3::launch_setup_mixed(g, h->params, h->ne_pad, h->d_geom, static_cast<float*>(h->d_Q), h->d_R,
                                              h->stream));
-            CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
+            if (h->timing) CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
             CP3H_HIP(cp3::launch_pairs_mixed(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_R,
                                              static_cast<float*>(h->d_out), h->stream));
         } else {
             CP3H_HIP(cp3::launch_geometry(g, h->d_geom, h->stream));
             CP3H_HIP(cp3::launch_spectrum_f64(g, h->params, h->ne_pad, static_cast<double*>(h->d_Q), h->d_R, h->stream));
-            CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
+            if (h->timing) CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
             CP3H_HIP(cp3::launch_pairs_f64(g, h->ne_pad, h->d_geom, static_cast<const double*>(h->d_Q), h->d_R,
                                            static_cast<double*>(h->d_out), h->stream));
         }
     }
-    CP3H_HIP(hipEventRecord(h->ev_stop, h->stream));
-    h->has_timing = true;
+    if (h->timing) CP3H_HIP(hipEventRecord(h->ev_stop, h->stream));
+    h->has_timing = h->timing;
     return CP3H_OK;
 }
 
@@ -426,14 +427,14 @@ int cp3h_compute_sqdiff_f32(cp3h_propagator* h, int antineutrino, double product
         return CP3H_OK;
     }
     if (!h->d_like) CP3H_HIP(hipMalloc(&h->d_like, sizeof(double) * blocks));
-    CP3H_HIP(hipEventRecord(h->ev_start, h->stream));
+    if (h->timing) CP3H_HIP(hipEventRecord(h->ev_start, h->stream));
     CP3H_HIP(cp3::launch_setup_mixed(g, h->params, h->ne_pad, h->d_geom, static_cast<float*>(h->d_Q), h->d_R,
                                      h->stream));
-    CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
+    if (h->timing) CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
     CP3H_HIP(cp3::launch_pairs_mixed_sqdiff(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_R,
                                             data_device, h->d_like, h->stream));
-    CP3H_HIP(hipEventRecord(h->ev_stop, h->stream));
-    h->has_timing = true;
+    if (h->timing) CP3H_HIP(hipEventRecord(h->ev_stop, h->stream));
+    h->has_timing = h->timing;
     CP3H_HIP(cp3::launch_sum_f64(h->d_like, blocks, h->d_partials, h->stream));
     CP3H_HIP(hipMemcpyAsync(h->h_scalar, h->d_partials + cp3::reduce_partials_needed(), sizeof(double),
                             hipMemcpyDeviceToHost, h->stream));
@@ -442,9 +443,16 @@ int cp3h_compute_sqdiff_f32(cp3h_propagator* h, int antineutrino, double product
     return CP3H_OK;
 }
 
+int cp3h_set_timing(cp3h_propagator* h, int enable) {
+    if (int rc = check_handle(h)) return rc;
+    h->timing = enable != 0;
+    return CP3H_OK;
+}
+
 int cp3h_last_compute_ms(cp3h_propagator* h, float* total_ms, float* pairs_ms) {
     if (int rc = check_handle(h)) return rc;
-    if (!h->has_timing) return fail(CP3H_ERR_STATE, "no compute has been launched");
+    if (!h->has_timing)
+        return fail(CP3H_ERR_STATE, "no compute has been launched with timing enabled (cp3h_set_timing)");
     CP3H_HIP(hipSetDevice(h->device));
     CP3H_HIP(hipEventSynchronize(h->ev_stop));
     if (total_ms) CP3H_HIP(hipEventElapsedTime(total_ms, h->ev_start, h->ev_stop));

@@ -2,11 +2,12 @@
 //
 // A batch owns concatenated device arrays for all sets (grids, Earth models, layer lengths,
 // mixed-precision spectrum tables) sized once at creation, float results allocated at the
-// first unbinned computation (binned mode writes bin averages to the caller's buffer), a pinned array
-// of per-set descriptors (hypothesis + offsets) that is copied to the device at each
-// computation, one stream, and optionally an instantiated HIP graph of
-// {descriptor copy, set-up kernel, persistent pair kernel}.  A parameter-scan step writes
-// new hypotheses into the pinned descriptors and replays the graph.
+// first unbinned computation (binned mode writes bin averages to the caller's buffer), two
+// pinned arrays of per-set descriptors (hypothesis + offsets), one of which is copied to the
+// device at each computation, one stream, and optionally instantiated HIP graphs of
+// {descriptor copy, set-up kernel, persistent pair kernel}, one per descriptor array.  A
+// parameter-scan step writes new hypotheses into the pending array while the device still
+// works on the previous step from the other array, then replays that array's graph.
 #include "../../include/cp3hip.h"
 #include "common.hpp"
 #include "host_util.hpp"
@@ -25,7 +26,10 @@ struct cp3h_batch {
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_pairs = nullptr, ev_stop = nullptr;
     std::vector<cp3::BatchSet> layout;
-    cp3::BatchSet* pinned = nullptr;
+    cp3::BatchSet* pinned = nullptr;           // the pending descriptors (= pinned_buf[next])
+    cp3::BatchSet* pinned_buf[2] = {nullptr, nullptr};
+    int next = 0;
+    hipEvent_t ev_done[2] = {nullptr, nullptr};  // end of the last computation from each array
     cp3::BatchSet* d_sets = nullptr;
     std::vector<long long> geom_prefix, spec_prefix, item_prefix;
     long long* d_prefix = nullptr;      // geom | spec | item prefixes, (n_sets + 1) each
@@ -49,7 +53,9 @@ struct cp3h_batch {
     int waves = 8, groups_per_cu = 4;
     bool use_graph = false;
     hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraphExec_t exec = nullptr;             // graph of the pending array (= execs[next])
+    hipGraph_t graphs[2] = {nullptr, nullptr};
+    hipGraphExec_t execs[2] = {nullptr, nullptr};
     bool launched = false, last_was_graph = false, ever_launched = false;
 
     cp3::BatchDesc desc() const {
@@ -97,8 +103,12 @@ int ensure_out(cp3h_batch* b) {
 }
 
 void drop_graph(cp3h_batch* b) {
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (int k = 0; k < 2; k++) {
+        if (b->execs[k]) (void)hipGraphExecDestroy(b->execs[k]);
+        if (b->graphs[k]) (void)hipGraphDestroy(b->graphs[k]);
+        b->execs[k] = nullptr;
+        b->graphs[k] = nullptr;
+    }
     b->exec = nullptr;
     b->graph = nullptr;
 }
@@ -132,8 +142,10 @@ int capture(cp3h_batch* b) {
         return rc;
     }
     if (end != hipSuccess) return fail(CP3H_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(end));
+    b->graphs[b->next] = g;
+    CP3H_HIP(hipGraphInstantiate(&b->execs[b->next], g, nullptr, nullptr, 0));
     b->graph = g;
-    CP3H_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->exec = b->execs[b->next];
     return CP3H_OK;
 }
 
@@ -198,8 +210,12 @@ int cp3h_batch_create(int device, int n_sets, const int* n_cosines, const int* n
         CP3H_HIP(hipEventCreate(&b->ev_start));
         CP3H_HIP(hipEventCreate(&b->ev_pairs));
         CP3H_HIP(hipEventCreate(&b->ev_stop));
-        CP3H_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->pinned), sizeof(cp3::BatchSet) * n_sets));
-        std::memcpy(b->pinned, b->layout.data(), sizeof(cp3::BatchSet) * n_sets);
+        for (int k = 0; k < 2; k++) {
+            CP3H_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->pinned_buf[k]), sizeof(cp3::BatchSet) * n_sets));
+            std::memcpy(b->pinned_buf[k], b->layout.data(), sizeof(cp3::BatchSet) * n_sets);
+            CP3H_HIP(hipEventCreateWithFlags(&b->ev_done[k], hipEventDisableTiming));
+        }
+        b->pinned = b->pinned_buf[0];
         CP3H_HIP(hipMalloc(&b->d_sets, sizeof(cp3::BatchSet) * n_sets));
         CP3H_HIP(hipMalloc(&b->d_prefix, sizeof(long long) * 3 * (n_sets + 1)));
         CP3H_HIP(hipMalloc(&b->d_cos, sizeof(double) * std::max(cos_n, 1LL)));
@@ -242,7 +258,10 @@ int cp3h_batch_destroy(cp3h_batch* b) {
     free_dev(b->d_next);
     free_dev(b->d_limits);
     free_dev(b->d_out);
-    if (b->pinned) (void)hipHostFree(b->pinned);
+    for (int k = 0; k < 2; k++) {
+        if (b->pinned_buf[k]) (void)hipHostFree(b->pinned_buf[k]);
+        if (b->ev_done[k]) (void)hipEventDestroy(b->ev_done[k]);
+    }
     if (b->ev_start) (void)hipEventDestroy(b->ev_start);
     if (b->ev_pairs) (void)hipEventDestroy(b->ev_pairs);
     if (b->ev_stop) (void)hipEventDestroy(b->ev_stop);
@@ -291,11 +310,7 @@ int cp3h_batch_set_params_f64(cp3h_batch* b, int set, const double* U18, const d
     if (!U18 || !DM9) return fail(CP3H_ERR_ARGUMENT, "null parameter arrays");
     if (antineutrino != 0 && antineutrino != 1) return fail(CP3H_ERR_ARGUMENT, "neutrino type must be 0 or 1");
     if (!(production_height_cm >= 0.0)) return fail(CP3H_ERR_ARGUMENT, "production height must be >= 0");
-    if (b->launched) {      // the pinned descriptors may still be in flight to the device
-        if (int rc = drain(b)) return rc;
-        b->launched = false;
-    }
-    cp3::BatchSet& s = b->pinned[set];
+    cp3::BatchSet& s = b->pinned[set];      // the pending array: no computation reads it
     std::memcpy(s.p.U, U18, sizeof(double) * 18);
     std::memcpy(s.p.DM, DM9, sizeof(double) * 9);
     cp3host::vacuum_order<double>(DM9, s.p.order);
@@ -347,6 +362,7 @@ int cp3h_batch_compute_async(cp3h_batch* b) {
     if (b->bin_nodes == 0)
         if (int rc = ensure_out(b)) return rc;
     CP3H_HIP(hipEventRecord(b->ev_start, b->stream));
+    b->exec = b->execs[b->next];
     if (b->use_graph) {
         if (!b->exec)
             if (int rc = capture(b)) return rc;
@@ -355,6 +371,15 @@ int cp3h_batch_compute_async(cp3h_batch* b) {
         if (int rc = enqueue(b, true)) return rc;
     }
     CP3H_HIP(hipEventRecord(b->ev_stop, b->stream));
+    // the other array becomes pending once the computation that last read it has finished (the
+    // one before this launch, so the device stays busy meanwhile); it starts as a copy of the
+    // descriptors just launched
+    CP3H_HIP(hipEventRecord(b->ev_done[b->next], b->stream));
+    const int q = 1 - b->next;
+    CP3H_HIP(hipEventSynchronize(b->ev_done[q]));
+    std::memcpy(b->pinned_buf[q], b->pinned_buf[b->next], sizeof(cp3::BatchSet) * b->n_sets);
+    b->next = q;
+    b->pinned = b->pinned_buf[q];
     b->last_was_graph = b->use_graph;
     b->launched = true;
     b->ever_launched = true;

@@ -361,6 +361,10 @@ class HipPropagatorSingle(Propagator):
         None returns to the propagator's own stream."""
         _lib.check(self._lib.cp3h_set_stream(self._handle(), ctypes.c_void_p(stream_ptr or None)))
 
+    def set_timing(self, enable: bool):
+        """Record HIP events around every compute (for last_compute_times); off by default."""
+        _lib.check(self._lib.cp3h_set_timing(self._handle(), int(bool(enable))))
+
     def last_compute_ms(self) -> float:
         """GPU milliseconds of all kernels of the last compute."""
         return self.last_compute_times()[0]

@@ -2,7 +2,7 @@
 and error map (cudaprob3_amd/data/surrogate_<earth>.npz).
 
     python -m cudaprob3_amd.train_surrogate [--earth jitter12|fine100] [--samples 50331648]
-                                            [--epochs 50] [--min-energy 2]
+                                            [--epochs 80] [--lr 1e-3] [--min-energy 2]
 
 Training data: `samples` points (cosZ, energy-bin centre, hypothesis) in sets of 64 random
 cosines x 64 log-uniform bin centres per random hypothesis (parameter ranges of the scan
@@ -51,9 +51,9 @@ def load(path: str = DEFAULT_PATH, max_batch: int = 8192, device: int = 0):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--samples", type=int, default=50_331_648)
-    ap.add_argument("--epochs", type=int, default=50)
+    ap.add_argument("--epochs", type=int, default=80)
     ap.add_argument("--batch", type=int, default=8192)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--min-energy", type=float, default=2.0)
     ap.add_argument("--seed", type=int, default=101)
     ap.add_argument("--earth", default="jitter12", choices=sorted(S.EARTHS))

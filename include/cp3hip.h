@@ -126,6 +126,10 @@ int cp3h_compute_sqdiff_f32(cp3h_propagator* h, int antineutrino, double product
    total_ms covers all of its kernels, pairs_ms only the per-pair propagation kernel
    (for CP3H_FP64_PLAIN the two are the same kernel).  Either pointer may be NULL. */
 int cp3h_last_compute_ms(cp3h_propagator* h, float* total_ms, float* pairs_ms);
+/* Timing of the computations (no counterpart in the reference): with enable != 0 every compute
+   records HIP events around its kernels for cp3h_last_compute_ms; off by default (each event
+   record adds a gap of several microseconds between the kernels). */
+int cp3h_set_timing(cp3h_propagator* h, int enable);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* cp3h_last_error(void);

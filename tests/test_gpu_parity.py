@@ -122,6 +122,10 @@ def test_api_errors_and_state():
         p.getProbability(4, 0, 0)
     v = p.getProbability(3, 3, 8)
     assert 0.0 <= v <= 1.0
+    with pytest.raises(RuntimeError, match="timing enabled"):
+        p.last_compute_ms()                 # timing is opt-in
+    p.set_timing(True)
+    p.calculateProbabilities(Neutrino)
     assert p.last_compute_ms() > 0.0
     p.close()
     with pytest.raises(RuntimeError, match="moved from"):
