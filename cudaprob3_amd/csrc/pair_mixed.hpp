@@ -50,96 +50,120 @@ __device__ inline void phase_factor_m1(double t, float& cm1, float& sn) {
     }
 }
 
+// The layer amplitude up to its global phase, A = I + w_1 Q_1 + w_2 Q_2 with
+// w_k = e^{i dphi_k} - 1; Q Hermitian, so A_ji uses conj(Q_ij).
+__device__ inline __attribute__((always_inline)) void layer_amplitude_mixed(double dist, const float4* __restrict__ qp,
+                                                                           const double2* __restrict__ rp,
+                                                                           std::size_t st, Cx<float> A[3][3]) {
+    float q[20];
+#pragma unroll
+    for (int grp = 0; grp < 5; grp++) {
+        const float4 v = qp[grp * st];
+        q[4 * grp] = v.x; q[4 * grp + 1] = v.y; q[4 * grp + 2] = v.z; q[4 * grp + 3] = v.w;
+    }
+    const double2 r = *rp;
+    float wr[2], wi[2];
+    phase_factor_m1(r.x * dist, wr[0], wi[0]);
+    phase_factor_m1(r.y * dist, wr[1], wi[1]);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        A[d][d].re = __builtin_fmaf(wr[1], q[9 + d], __builtin_fmaf(wr[0], q[d], 1.0f));
+        A[d][d].im = __builtin_fmaf(wi[1], q[9 + d], wi[0] * q[d]);
+    }
+    const int idx[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+#pragma unroll
+    for (int w = 0; w < 3; w++) {
+        const float a0 = q[3 + 2 * w], b0 = q[4 + 2 * w], a1 = q[12 + 2 * w], b1 = q[13 + 2 * w];
+        // w_k (a + i b) and w_k (a - i b), summed over k
+        float ure = wr[0] * a0, uim = wr[0] * b0, lre = ure, lim = wi[0] * a0;
+        ure = __builtin_fmaf(-wi[0], b0, ure);
+        uim = __builtin_fmaf(wi[0], a0, uim);
+        lre = __builtin_fmaf(wi[0], b0, lre);
+        lim = __builtin_fmaf(-wr[0], b0, lim);
+        ure = __builtin_fmaf(wr[1], a1, ure);
+        ure = __builtin_fmaf(-wi[1], b1, ure);
+        uim = __builtin_fmaf(wr[1], b1, uim);
+        uim = __builtin_fmaf(wi[1], a1, uim);
+        lre = __builtin_fmaf(wr[1], a1, lre);
+        lre = __builtin_fmaf(wi[1], b1, lre);
+        lim = __builtin_fmaf(wi[1], a1, lim);
+        lim = __builtin_fmaf(-wr[1], b1, lim);
+        A[idx[w][0]][idx[w][1]] = Cx<float>{ure, uim};
+        A[idx[w][1]][idx[w][0]] = Cx<float>{lre, lim};
+    }
+}
+
+// F <- A F (F: columns 0-1 of the forward product)
+__device__ inline __attribute__((always_inline)) void left_mul_cols(const Cx<float> A[3][3], Cx<float> F[3][2]) {
+    Cx<float> G[3][2];
+#pragma unroll
+    for (int x = 0; x < 3; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) {
+            Cx<float> acc = cmul(A[x][0], F[0][y]);
+            cmac(acc, A[x][1], F[1][y]);
+            cmac(acc, A[x][2], F[2][y]);
+            G[x][y] = acc;
+        }
+#pragma unroll
+    for (int x = 0; x < 3; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) F[x][y] = G[x][y];
+}
+
+// M <- M A (M: rows 0-1 of the outgoing mantle product)
+__device__ inline __attribute__((always_inline)) void right_mul_rows(Cx<float> M[2][3], const Cx<float> A[3][3]) {
+    Cx<float> N[2][3];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 3; y++) {
+            Cx<float> acc = cmul(M[x][0], A[0][y]);
+            cmac(acc, M[x][1], A[1][y]);
+            cmac(acc, M[x][2], A[2][y]);
+            N[x][y] = acc;
+        }
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 3; y++) M[x][y] = N[x][y];
+}
+
+// The path's layers in the reference's order (physics.hpp:645-677): layer 0 (atmosphere), then
+// shells 1..L-1 on the way in, which enter both the forward product and the mantle product
+// of the way out, then the innermost shell L, crossed once; total = mantle x forward.  The
+// first two layers and the last are peeled off the loop, so its body has no branches; L is
+// wave-uniform, so every branch here is a scalar one.
 __device__ inline void propagate_pair_mixed(int L, const double* __restrict__ gl, const float4* __restrict__ Q,
                                             const double2* __restrict__ R, std::size_t st, int ie,
                                             float p[3][3]) {
+    const float4* qp = Q + ie;
+    const double2* rp = R + ie;
+    Cx<float> A[3][3];
     Cx<float> F[3][2];   // columns 0-1 of T_i ... T_1 T_atm
-    Cx<float> M[2][3];   // rows 0-1 of T_1 ... T_{i}
-    for (int i = 0; i <= L; i++) {
-        const double dist = gl[i];
-        const float4* qp = Q + std::size_t(i) * 5 * st + ie;
-        float q[20];
+    Cx<float> M[2][3];   // rows 0-1 of T_1 ... T_i
+    layer_amplitude_mixed(gl[0], qp, rp, st, A);
 #pragma unroll
-        for (int grp = 0; grp < 5; grp++) {
-            const float4 v = qp[grp * st];
-            q[4 * grp] = v.x; q[4 * grp + 1] = v.y; q[4 * grp + 2] = v.z; q[4 * grp + 3] = v.w;
+    for (int x = 0; x < 3; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) F[x][y] = A[x][y];
+    if (L >= 2) {
+        layer_amplitude_mixed(gl[1], qp + 5 * st, rp + st, st, A);
+        left_mul_cols(A, F);
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 3; y++) M[x][y] = A[x][y];
+#pragma unroll 2
+        for (int i = 2; i < L; i++) {
+            layer_amplitude_mixed(gl[i], qp + std::size_t(i) * 5 * st, rp + std::size_t(i) * st, st, A);
+            left_mul_cols(A, F);
+            right_mul_rows(M, A);
         }
-        const double2 r = R[std::size_t(i) * st + ie];
-        float wr[2], wi[2];
-        phase_factor_m1(r.x * dist, wr[0], wi[0]);
-        phase_factor_m1(r.y * dist, wr[1], wi[1]);
-        // A = I + w_1 Q_1 + w_2 Q_2; Q Hermitian: A_ji uses conj(Q_ij)
-        Cx<float> A[3][3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            A[d][d].re = __builtin_fmaf(wr[1], q[9 + d], __builtin_fmaf(wr[0], q[d], 1.0f));
-            A[d][d].im = __builtin_fmaf(wi[1], q[9 + d], wi[0] * q[d]);
-        }
-        const int idx[3][2] = {{0, 1}, {0, 2}, {1, 2}};
-#pragma unroll
-        for (int w = 0; w < 3; w++) {
-            const float a0 = q[3 + 2 * w], b0 = q[4 + 2 * w], a1 = q[12 + 2 * w], b1 = q[13 + 2 * w];
-            // w_k (a + i b) and w_k (a - i b), summed over k
-            float ure = wr[0] * a0, uim = wr[0] * b0, lre = ure, lim = wi[0] * a0;
-            ure = __builtin_fmaf(-wi[0], b0, ure);
-            uim = __builtin_fmaf(wi[0], a0, uim);
-            lre = __builtin_fmaf(wi[0], b0, lre);
-            lim = __builtin_fmaf(-wr[0], b0, lim);
-            ure = __builtin_fmaf(wr[1], a1, ure);
-            ure = __builtin_fmaf(-wi[1], b1, ure);
-            uim = __builtin_fmaf(wr[1], b1, uim);
-            uim = __builtin_fmaf(wi[1], a1, uim);
-            lre = __builtin_fmaf(wr[1], a1, lre);
-            lre = __builtin_fmaf(wi[1], b1, lre);
-            lim = __builtin_fmaf(wi[1], a1, lim);
-            lim = __builtin_fmaf(-wr[1], b1, lim);
-            A[idx[w][0]][idx[w][1]] = Cx<float>{ure, uim};
-            A[idx[w][1]][idx[w][0]] = Cx<float>{lre, lim};
-        }
-        if (i == 0) {
-#pragma unroll
-            for (int x = 0; x < 3; x++)
-#pragma unroll
-                for (int y = 0; y < 2; y++) F[x][y] = A[x][y];
-        } else {
-            Cx<float> G[3][2];
-#pragma unroll
-            for (int x = 0; x < 3; x++)
-#pragma unroll
-                for (int y = 0; y < 2; y++) {
-                    Cx<float> acc = cmul(A[x][0], F[0][y]);
-                    cmac(acc, A[x][1], F[1][y]);
-                    cmac(acc, A[x][2], F[2][y]);
-                    G[x][y] = acc;
-                }
-#pragma unroll
-            for (int x = 0; x < 3; x++)
-#pragma unroll
-                for (int y = 0; y < 2; y++) F[x][y] = G[x][y];
-            if (i < L) {
-                if (i == 1) {
-#pragma unroll
-                    for (int x = 0; x < 2; x++)
-#pragma unroll
-                        for (int y = 0; y < 3; y++) M[x][y] = A[x][y];
-                } else {
-                    Cx<float> N[2][3];
-#pragma unroll
-                    for (int x = 0; x < 2; x++)
-#pragma unroll
-                        for (int y = 0; y < 3; y++) {
-                            Cx<float> acc = cmul(M[x][0], A[0][y]);
-                            cmac(acc, M[x][1], A[1][y]);
-                            cmac(acc, M[x][2], A[2][y]);
-                            N[x][y] = acc;
-                        }
-#pragma unroll
-                    for (int x = 0; x < 2; x++)
-#pragma unroll
-                        for (int y = 0; y < 3; y++) M[x][y] = N[x][y];
-                }
-            }
-        }
+    }
+    if (L >= 1) {
+        layer_amplitude_mixed(gl[L], qp + std::size_t(L) * 5 * st, rp + std::size_t(L) * st, st, A);
+        left_mul_cols(A, F);
     }
     // upper-left 2x2 block S[out][in] of the total amplitude
     Cx<float> S[2][2];

@@ -27,7 +27,8 @@
 // Mapping: one wavefront = one cosine row x 64 consecutive energies (lane = energy); four
 // wavefronts of a workgroup take four consecutive rows of the same energy tile, and
 // consecutive logical workgroups stay on one XCD so that an energy tile's table slice is
-// reused from that XCD's L2.  Layer lengths of the row are wave-uniform scalar loads.
+// reused from that XCD's L2.  The row index is made wave-uniform (readfirstlane), so the
+// layer loop runs on scalar control flow.
 #include "common.hpp"
 #include "pair_mixed.hpp"
 #include "reduce.hpp"
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void pairs_mixed_kernel(MixedArgs a) {
     const int tiles = (a.n_e + kWave - 1) / kWave;
     const int row_groups = (a.n_c + 3) / 4;
     const int tile = lb / row_groups;
-    const int ic = (lb % row_groups) * 4 + threadIdx.x / kWave;
+    const int ic = __builtin_amdgcn_readfirstlane(int((lb % row_groups) * 4 + threadIdx.x / kWave));
     const bool valid = tile < tiles && ic < a.n_c;
     if (!kLike && !valid) return;
     const int ie = tile * kWave + threadIdx.x % kWave;   // < ne_pad: table reads stay in bounds
