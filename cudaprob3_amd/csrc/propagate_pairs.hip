@@ -24,8 +24,8 @@
 //    mantle layer instead of 27 + 27.
 //  * Q_1, Q_2 are read as five coalesced float4 loads per lane and layer.
 //
-// Mapping: one wavefront = one cosine row x 64 consecutive energies (lane = energy); four
-// wavefronts of a workgroup take four consecutive rows of the same energy tile, and
+// Mapping: one wavefront = two cosine rows x 64 consecutive energies (lane = energy); the four
+// wavefronts of a workgroup take eight consecutive rows of the same energy tile, and
 // consecutive logical workgroups stay on one XCD so that an energy tile's table slice is
 // reused from that XCD's L2.  The row index is made wave-uniform (readfirstlane), so the
 // layer loop runs on scalar control flow.
@@ -187,40 +187,57 @@ struct MixedArgs {
 // kLike: instead of storing P, sum (P - data)^2 in FP64 over the workgroup's pairs (the
 // Markov-chain likelihood without a result round trip through HBM); every wavefront takes
 // part in the workgroup reduction, so none returns early.
+// Each wavefront takes two consecutive cosine rows (kRowsPerWave) at its 64 energies: rows with the
+// same number of crossed shells -- all but the few pairs that straddle a shell boundary -- are
+// propagated together and share every table read; otherwise one after the other.
 template <bool kLike>
 __global__ __launch_bounds__(kBlock) void pairs_mixed_kernel(MixedArgs a) {
     const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
     const int tiles = (a.n_e + kWave - 1) / kWave;
-    const int row_groups = (a.n_c + 3) / 4;
+    constexpr int kRowsPerBlock = kRowsPerWave * (kBlock / kWave);
+    const int row_groups = (a.n_c + kRowsPerBlock - 1) / kRowsPerBlock;
     const int tile = lb / row_groups;
-    const int ic = __builtin_amdgcn_readfirstlane(int((lb % row_groups) * 4 + threadIdx.x / kWave));
-    const bool valid = tile < tiles && ic < a.n_c;
-    if (!kLike && !valid) return;
+    const int ic0 =
+        __builtin_amdgcn_readfirstlane(int((lb % row_groups) * kRowsPerBlock + (threadIdx.x / kWave) * kRowsPerWave));
+    const int n_rows = tile < tiles ? max(0, min(kRowsPerWave, a.n_c - ic0)) : 0;
+    if (!kLike && n_rows == 0) return;
     const int ie = tile * kWave + threadIdx.x % kWave;   // < ne_pad: table reads stay in bounds
     const std::size_t plane = std::size_t(a.n_c) * a.n_e;
+    const std::size_t st = a.ne_pad;
 
     double acc = 0.0;
-    if (valid) {
-        const int L = a.maxlayers[ic];
-        const double* gl = a.geom + std::size_t(ic) * (a.rows + 1);
-        const std::size_t st = a.ne_pad;
-        float p[3][3];
-        propagate_pair_mixed(L, gl, a.Q, a.R, st, ie, p);
-        if (ie < a.n_e) {
-            if (kLike) {
-                const float* d = a.data + std::size_t(ic) * a.n_e + ie;
+    auto emit = [&](int ic, const float (&p)[3][3]) {
+        if (ie >= a.n_e) return;
+        if (kLike) {
+            const float* d = a.data + std::size_t(ic) * a.n_e + ie;
 #pragma unroll
-                for (int t = 0; t < 9; t++) {
-                    const double e = double(p[t / 3][t % 3]) - double(d[t * plane]);
-                    acc = __builtin_fma(e, e, acc);
-                }
-            } else {
-                float* o = a.out + std::size_t(ic) * a.n_e + ie;
-#pragma unroll
-                for (int in = 0; in < 3; in++)
-#pragma unroll
-                    for (int ou = 0; ou < 3; ou++) o[(in * 3 + ou) * plane] = p[in][ou];
+            for (int t = 0; t < 9; t++) {
+                const double e = double(p[t / 3][t % 3]) - double(d[t * plane]);
+                acc = __builtin_fma(e, e, acc);
             }
+        } else {
+            float* o = a.out + std::size_t(ic) * a.n_e + ie;
+#pragma unroll
+            for (int in = 0; in < 3; in++)
+#pragma unroll
+                for (int ou = 0; ou < 3; ou++) o[(in * 3 + ou) * plane] = p[in][ou];
+        }
+    };
+    const int L0 = n_rows > 0 ? a.maxlayers[ic0] : 0;
+    if (n_rows == 2 && a.maxlayers[ic0 + 1] == L0) {
+        const double* gl[2] = {a.geom + std::size_t(ic0) * (a.rows + 1), a.geom + std::size_t(ic0 + 1) * (a.rows + 1)};
+        float p[2][3][3];
+        propagate_rows_mixed<2>(L0, gl, a.Q, a.R, st, ie, p);
+        emit(ic0, p[0]);
+        emit(ic0 + 1, p[1]);
+    } else {
+#pragma unroll 1
+        for (int k = 0; k < n_rows; k++) {
+            const int ic = ic0 + k;
+            const double* gl[1] = {a.geom + std::size_t(ic) * (a.rows + 1)};
+            float p[1][3][3];
+            propagate_rows_mixed<1>(a.maxlayers[ic], gl, a.Q, a.R, st, ie, p);
+            emit(ic, p[0]);
         }
     }
     if (kLike) {
@@ -245,7 +262,8 @@ hipError_t launch_pairs_fp64(const GridDesc& g, int ne_pad, const double* geom, 
 unsigned likelihood_blocks(const GridDesc& g) {
     if (g.n_c == 0 || g.n_e == 0) return 0;
     const unsigned tiles = (g.n_e + kWave - 1) / kWave;
-    return round_up(tiles * ((g.n_c + 3) / 4), kXcds);
+    constexpr int kRowsPerBlock = kRowsPerWave * (kBlock / kWave);
+    return round_up(tiles * ((g.n_c + kRowsPerBlock - 1) / kRowsPerBlock), kXcds);
 }
 
 hipError_t launch_pairs_mixed(const GridDesc& g, int ne_pad, const double* geom, const float* Q,
