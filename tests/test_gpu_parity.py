@@ -84,6 +84,27 @@ def test_ragged_and_empty_grids(oracle_mod, precision, n_c, n_e):
         assert np.abs(P - ref).max() <= tol
 
 
+def test_mixed_rows_together_equal_rows_alone():
+    """The mixed kernel propagates two cosine rows per wavefront together when they cross the
+    same number of shells (sharing table reads) and one after the other otherwise; every row of a
+    grid must equal, bit for bit, the same row propagated alone (a one-row grid).  The rows come
+    in pairs (a wavefront's two rows): for every shell count L = 0..12 two rows inside the same
+    band, then pairs straddling each band edge, then an odd last row."""
+    radii = W.PREM12[:, 0]
+    R = radii.max()
+    inner = np.sort(radii[(radii > 0) & (radii < R)])[::-1]
+    edges = np.concatenate([[0.0], -np.sqrt(1 - (inner / R) ** 2), [-1.0]])   # L = k for edges[k+1] <= cos < edges[k]
+    same = [x for k in range(len(edges) - 1) for x in np.linspace(edges[k + 1], edges[k], 4)[1:3]]
+    straddle = [x for e in edges[1:-1] for x in (e - 1e-4, e + 1e-4)]
+    cos = np.array([0.5, 0.7] + same + straddle + [-0.33])
+    E = np.geomspace(0.1, 100, 70)
+    print(f"[gpu] mixed rows together vs alone: {len(cos)} rows x {len(E)}", flush=True)
+    full = run_gpu("mixed", cos, E, W.PREM12[:, 0], W.PREM12[:, 1], W.NUFIT, 22.0, 0)
+    for i, c in enumerate(cos):
+        alone = run_gpu("mixed", np.array([c]), E, W.PREM12[:, 0], W.PREM12[:, 1], W.NUFIT, 22.0, 0)
+        np.testing.assert_array_equal(full[i], alone[0])
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_jitter12_2048_full_size(oracle_mod, precision):
     """The 2048 x 2048 jittered 12-shell workload, all four nu/nubar x NO/IO variants:
