@@ -6,8 +6,8 @@
 // bin, reduces its weighted probabilities with butterfly lane exchanges, and its first lane
 // stores the nine bin averages (16x less output than the unbinned results).
 //
-// Work items are (set, 64-energy tile, cosine row), numbered set-major, then tile-major, so
-// consecutive items share a spectrum-table slice.  A fixed grid of workgroups (sized from
+// Work items are (set, 64-energy tile, pair of cosine rows), numbered set-major, then
+// tile-major, so consecutive items share a spectrum-table slice.  A fixed grid of workgroups (sized from
 // the CU count by the autotuner) drains the item list through work queues, one per XCD:
 // the list is cut into kXcds contiguous parts, and the wavefronts of XCD x (workgroups are
 // dispatched to XCDs round robin, so x = blockIdx % kXcds) take kGrab consecutive items at a
@@ -47,19 +47,11 @@ __device__ inline long long first_lane(unsigned long long v) {
     return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
-// one work item: 64 energies (one per lane) of one cosine row (local = tile * n_c + row) of set k
+// Stores one row's results: the nine P at this lane's energy, or, in binned mode, the nine bin
+// averages of each aligned group of `nodes` lanes (every lane takes part in the exchanges).
 template <bool kBinned>
-__device__ inline void pair_item(const BatchDesc& b, int k, long long local, int lane) {
-    const BatchSet& set = b.sets[k];
-    const int tile = int(local / set.n_c);
-    const int ic = int(local % set.n_c);
-    const int ie = tile * kWave + lane;            // < ne_pad
-    const int n_e = set.has_limits ? min(set.n_e, b.row_limit[set.cos_off + ic]) : set.n_e;
-    if (tile * kWave >= n_e) return;               // the row's energies end before this tile
-    const int L = b.maxlayers[set.cos_off + ic];
-    const double* gl = b.geom + set.geom_off + std::size_t(ic) * (set.rows + 1);
-    float p[3][3];
-    propagate_pair_mixed(L, gl, b.Q + set.tab_off * 5, b.R + set.tab_off, set.ne_pad, ie, p);
+__device__ inline void store_row(const BatchDesc& b, const BatchSet& set, int ic, int n_e, int ie, int lane,
+                                 const float (&p)[3][3]) {
     if (kBinned) {
         // groups of `nodes` lanes are whole bins, all inside or all beyond the set's n_e (a row
         // limit inside a bin still leaves every node of that bin computed: the tile is whole)
@@ -83,6 +75,48 @@ __device__ inline void pair_item(const BatchDesc& b, int k, long long local, int
         for (int in = 0; in < 3; in++)
 #pragma unroll
             for (int ou = 0; ou < 3; ou++) o[(in * 3 + ou) * plane] = p[in][ou];
+    }
+}
+
+// One work item: 64 energies (one per lane) of a pair of cosine rows (rows 2j, 2j+1; local =
+// tile * pairs + j) of set k.  Two rows that both reach this tile and cross the same number of
+// shells are propagated together and share every table read (set tables are larger than L1,
+// so a row on its own re-reads its slice from L2); otherwise each on its own.
+template <bool kBinned>
+__device__ inline void pair_item(const BatchDesc& b, int k, long long local, int lane) {
+    const BatchSet& set = b.sets[k];
+    const int pairs = (set.n_c + kRowsPerWave - 1) / kRowsPerWave;
+    const int tile = int(local / pairs);
+    const int ic0 = int(local % pairs) * kRowsPerWave;
+    const int ie = tile * kWave + lane;            // < ne_pad
+    auto row_ne = [&](int ic) {                    // the row's energies (a row limit ends them early)
+        return set.has_limits ? min(set.n_e, b.row_limit[set.cos_off + ic]) : set.n_e;
+    };
+    const int ne0 = row_ne(ic0);
+    const int ne1 = ic0 + 1 < set.n_c ? row_ne(ic0 + 1) : 0;
+    const bool v0 = tile * kWave < ne0, v1 = tile * kWave < ne1;
+    const std::size_t gstride = set.rows + 1;
+    const double* gbase = b.geom + set.geom_off;
+    const float4* Q = b.Q + set.tab_off * 5;
+    const double2* R = b.R + set.tab_off;
+    const int L0 = v0 ? b.maxlayers[set.cos_off + ic0] : -1;
+    const int L1 = v1 ? b.maxlayers[set.cos_off + ic0 + 1] : -2;
+    if (L0 == L1) {
+        const double* gl[2] = {gbase + std::size_t(ic0) * gstride, gbase + std::size_t(ic0 + 1) * gstride};
+        float p[2][3][3];
+        propagate_rows_mixed<2>(L0, gl, Q, R, set.ne_pad, ie, p);
+        store_row<kBinned>(b, set, ic0, ne0, ie, lane, p[0]);
+        store_row<kBinned>(b, set, ic0 + 1, ne1, ie, lane, p[1]);
+        return;
+    }
+#pragma unroll 1
+    for (int r = 0; r < 2; r++) {
+        if (!(r == 0 ? v0 : v1)) continue;
+        const int ic = ic0 + r;
+        const double* gl[1] = {gbase + std::size_t(ic) * gstride};
+        float p[1][3][3];
+        propagate_rows_mixed<1>(r == 0 ? L0 : L1, gl, Q, R, set.ne_pad, ie, p);
+        store_row<kBinned>(b, set, ic, r == 0 ? ne0 : ne1, ie, lane, p[0]);
     }
 }
 

@@ -27,7 +27,7 @@ struct OscParams {
 
 constexpr int kWave = 64;            // CDNA wavefront
 constexpr int kBlock = 256;          // 4 wavefronts per workgroup
-constexpr int kRowsPerWave = 2;      // cosine rows per wavefront in the single-grid mixed kernel
+constexpr int kRowsPerWave = 2;      // cosine rows per wavefront (single-grid) or work item (batched), mixed kernels
 constexpr int kXcds = 8;             // MI355X accelerator complex dies
 
 // Workgroup id -> logical id so that consecutive logical ids run on the same XCD

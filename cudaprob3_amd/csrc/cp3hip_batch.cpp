@@ -200,7 +200,8 @@ int cp3h_batch_create(int device, int n_sets, const int* n_cosines, const int* n
         out_n += 9LL * s.n_c * s.n_e;
         b->geom_prefix[i + 1] = b->geom_prefix[i] + geom;
         b->spec_prefix[i + 1] = b->spec_prefix[i] + (s.n_e > 0 ? cells : 0);
-        b->item_prefix[i + 1] = b->item_prefix[i] + (long long)s.n_c * ((s.n_e + cp3::kWave - 1) / cp3::kWave);
+        b->item_prefix[i + 1] = b->item_prefix[i] + (long long)((s.n_c + cp3::kRowsPerWave - 1) / cp3::kRowsPerWave) *
+                                                    ((s.n_e + cp3::kWave - 1) / cp3::kWave);
     }
     auto bail = [&](int code) { cp3h_batch_destroy(b); return code; };
     const int rc = [&]() -> int {
