@@ -186,7 +186,8 @@ __device__ __attribute__((always_inline)) inline void fused_tile(const FusedArgs
     }
 
     // scaled inputs as two terms: hi in columns 0..8, lo in kMlpInLo.., zeros up to column 31
-    for (int e = threadIdx.x; e < kTileRows * kMlpInPad; e += blockDim.x) {
+#pragma unroll 4
+    for (int e = threadIdx.x; e < kTileRows * kMlpInPad; e += 256 * kRowGroups) {    // blockDim.x = 256 * kRowGroups
         const int r = e / kMlpInPad, c = e % kMlpInPad;
         const int src = c < kMlpIn ? c : c - kMlpInLo;
         float v = 0.0f;
