@@ -184,12 +184,14 @@ struct MixedArgs {
     int n_c, n_e, rows, ne_pad;
 };
 
-// kLike: instead of storing P, sum (P - data)^2 in FP64 over the workgroup's pairs (the
-// Markov-chain likelihood without a result round trip through HBM); every wavefront takes
-// part in the workgroup reduction, so none returns early.
 // Each wavefront takes two consecutive cosine rows (kRowsPerWave) at its 64 energies: rows with the
 // same number of crossed shells -- all but the few pairs that straddle a shell boundary -- are
 // propagated together and share every table read; otherwise one after the other.
+//
+// kLike: instead of storing P, sum (P - data)^2 in FP64 over the workgroup's pairs (the
+// Markov-chain likelihood without a result round trip through HBM); every wavefront takes
+// part in the workgroup reduction, so none returns early.
+static_assert(kRowsPerWave == 2, "the mixed kernels pair rows two by two");
 template <bool kLike>
 __global__ __launch_bounds__(kBlock) void pairs_mixed_kernel(MixedArgs a) {
     const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
