@@ -69,12 +69,14 @@ __device__ inline void store_row(const BatchDesc& b, const BatchSet& set, int ic
             for (int t = 0; t < 9; t++) o[t * set.bin_plane] = v[t];
         }
     } else if (ie < n_e) {
+        // non-temporal: the results stream past L2 instead of evicting the table slices that the
+        // other rows of the tile still read
         const std::size_t plane = std::size_t(set.n_c) * set.n_e;
         float* o = b.out + set.out_off + std::size_t(ic) * set.n_e + ie;
 #pragma unroll
         for (int in = 0; in < 3; in++)
 #pragma unroll
-            for (int ou = 0; ou < 3; ou++) o[(in * 3 + ou) * plane] = p[in][ou];
+            for (int ou = 0; ou < 3; ou++) __builtin_nontemporal_store(p[in][ou], o + (in * 3 + ou) * plane);
     }
 }
 

@@ -218,11 +218,11 @@ __global__ __launch_bounds__(kBlock) void pairs_mixed_kernel(MixedArgs a) {
                 acc = __builtin_fma(e, e, acc);
             }
         } else {
-            float* o = a.out + std::size_t(ic) * a.n_e + ie;
+            float* o = a.out + std::size_t(ic) * a.n_e + ie;     // non-temporal: results stream past L2
 #pragma unroll
             for (int in = 0; in < 3; in++)
 #pragma unroll
-                for (int ou = 0; ou < 3; ou++) o[(in * 3 + ou) * plane] = p[in][ou];
+                for (int ou = 0; ou < 3; ou++) __builtin_nontemporal_store(p[in][ou], o + (in * 3 + ou) * plane);
         }
     };
     const int L0 = n_rows > 0 ? a.maxlayers[ic0] : 0;
