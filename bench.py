@@ -225,7 +225,8 @@ def run_mcmc(args, sync):
         "latency_ms": {"p50": p50, "p99": p99, "mean": float(np.mean(st.step_ms))},
         "acceptance": st.accepted / max(st.steps, 1),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": M.PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / M.PEAK_FP32_VALU_TFLOPS, "traffic": None, "kernel": "pairs_mixed_kernel",
+                     "frac": achieved / M.PEAK_FP32_VALU_TFLOPS, "traffic": traffic_from_profile("mcmc8192", "mixed"),
+                     "kernel": "pairs_mixed_kernel",
                      "kernel_ms": k_ms, "all_kernels_ms": total_ms, "algorithmic_flops_per_launch": flops},
     }
     chain.close()
