@@ -71,6 +71,8 @@ SIGNATURES = {
     "cp3h_batch_last_ms": (ctypes.c_int, [_handle, _c_float_p, _c_float_p]),
     "cp3h_batch_copy_results_f32": (ctypes.c_int, [_handle, ctypes.c_int, _c_float_p, ctypes.c_size_t]),
     "cp3h_batch_device_results": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "cp3h_batch_result_strides": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),
+                                                 ctypes.POINTER(ctypes.c_int)]),
     "cp3h_batch_bin_average_f32": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_float_p,
                                                   ctypes.c_void_p]),
     "cp3h_batch_set_binning": (ctypes.c_int, [_handle, ctypes.c_int, _c_float_p, ctypes.c_void_p,

@@ -200,9 +200,17 @@ class BatchPropagator:
         return out
 
     def device_results(self, i: int) -> int:
+        """Device address of set i's results: P(t, ic, ie) at [t * plane + ic * row + ie] floats,
+        (plane, row) = result_strides(i)."""
         p = ctypes.c_void_p()
         _lib.check(self._lib.cp3h_batch_device_results(self._h, i, ctypes.byref(p)))
         return p.value
+
+    def result_strides(self, i: int) -> tuple[int, int]:
+        """(plane, row) strides in floats of set i's device results (rows padded to 32 floats)."""
+        plane, row = ctypes.c_longlong(), ctypes.c_int()
+        _lib.check(self._lib.cp3h_batch_result_strides(self._h, i, ctypes.byref(plane), ctypes.byref(row)))
+        return plane.value, row.value
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

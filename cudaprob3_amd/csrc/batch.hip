@@ -71,8 +71,8 @@ __device__ inline void store_row(const BatchDesc& b, const BatchSet& set, int ic
     } else if (ie < n_e) {
         // non-temporal: the results stream past L2 instead of evicting the table slices that the
         // other rows of the tile still read
-        const std::size_t plane = std::size_t(set.n_c) * set.n_e;
-        float* o = b.out + set.out_off + std::size_t(ic) * set.n_e + ie;
+        const std::size_t plane = set.out_plane;
+        float* o = b.out + set.out_off + std::size_t(ic) * set.out_row + ie;
 #pragma unroll
         for (int in = 0; in < 3; in++)
 #pragma unroll
@@ -156,16 +156,19 @@ struct BinWeights {
     float w[kMaxBinNodes];
 };
 
-// out[(t * n_c + ic) * n_bins + b] = sum_g w[g] * P[t][ic][b * nodes + g] for one set
-__global__ __launch_bounds__(kBlock) void bin_average(const float* __restrict__ P, int n_c, int n_e, int nodes,
-                                                      BinWeights w, float* __restrict__ out) {
+// out[(t * n_c + ic) * n_bins + b] = sum_g w[g] * P(t, ic, b * nodes + g) for one set, whose
+// results sit at P[t * plane + ic * row + ie]
+__global__ __launch_bounds__(kBlock) void bin_average(const float* __restrict__ P, int n_c, int n_e, int row_stride,
+                                                      long long plane, int nodes, BinWeights w,
+                                                      float* __restrict__ out) {
     const int n_bins = n_e / nodes;
     const long long total = 9LL * n_c * n_bins;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
          i += (long long)gridDim.x * blockDim.x) {
         const long long row = i / n_bins;                 // t * n_c + ic
         const int bin = int(i % n_bins);
-        const float* src = P + row * n_e + (long long)bin * nodes;
+        const long long t = row / n_c, ic = row - t * n_c;
+        const float* src = P + t * plane + ic * row_stride + (long long)bin * nodes;
         float acc = 0.0f;
         for (int g = 0; g < nodes; g++) acc = __builtin_fmaf(w.w[g], src[g], acc);
         out[i] = acc;
@@ -174,15 +177,15 @@ __global__ __launch_bounds__(kBlock) void bin_average(const float* __restrict__ 
 
 }  // namespace
 
-hipError_t launch_bin_average(const float* P, int n_c, int n_e, int nodes, const float* weights, float* out,
-                              hipStream_t s) {
+hipError_t launch_bin_average(const float* P, int n_c, int n_e, int row, long long plane, int nodes,
+                              const float* weights, float* out, hipStream_t s) {
     if (nodes < 1 || nodes > kMaxBinNodes || n_e % nodes != 0) return hipErrorInvalidValue;
     BinWeights w{};
     for (int g = 0; g < nodes; g++) w.w[g] = weights[g];
     const long long total = 9LL * n_c * (n_e / nodes);
     if (total == 0) return hipSuccess;
     const unsigned blocks = unsigned(std::min<long long>((total + kBlock - 1) / kBlock, 65535));
-    hipLaunchKernelGGL(bin_average, dim3(blocks), dim3(kBlock), 0, s, P, n_c, n_e, nodes, w, out);
+    hipLaunchKernelGGL(bin_average, dim3(blocks), dim3(kBlock), 0, s, P, n_c, n_e, row, plane, nodes, w, out);
     return hipGetLastError();
 }
 

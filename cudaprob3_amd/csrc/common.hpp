@@ -107,7 +107,9 @@ struct BatchSet {
     long long model_off;  // radii / rhos (descending)
     long long geom_off;   // layer lengths, n_c * (rows + 1)
     long long tab_off;    // spectrum cells, (rows + 1) * ne_pad
-    long long out_off;    // results, 9 * n_c * n_e floats (SoA per set)
+    long long out_off;    // results: P(t, ic, ie) at out[out_off + t * out_plane + ic * out_row + ie]
+    long long out_plane;  // n_c * out_row
+    int out_row;          // n_e rounded up to kOutRowAlign floats, so that every row starts a cache line
     // binned mode: bin average (t, ic, bin) -> bin_out[bin_off + t * bin_plane + ic * bin_row + bin]
     long long bin_off, bin_plane;
     int bin_row;
@@ -116,6 +118,7 @@ struct BatchSet {
 };
 
 constexpr int kMaxBinNodes = 32;
+constexpr int kOutRowAlign = 32;     // floats: 128 bytes, one cache line
 // work-queue counters of the batched pair kernel, one per XCD, each in its own 256-byte line
 constexpr int kQueueStride = 32;
 
@@ -155,9 +158,10 @@ hipError_t launch_batch_setup(const BatchDesc& b, int blocks, hipStream_t s);
 // weighted bin averages are stored.
 hipError_t launch_batch_pairs(const BatchDesc& b, int blocks, int waves, hipStream_t s);
 
-// Energy-bin average of one set's SoA results whose n_e energies are `nodes` quadrature nodes
-// per bin: out[9][n_c][n_e / nodes].
-hipError_t launch_bin_average(const float* P, int n_c, int n_e, int nodes, const float* weights, float* out,
+// Energy-bin average of one set's results (P(t, ic, ie) at P[t * plane + ic * row + ie]) whose
+// n_e energies are `nodes` quadrature nodes per bin: out[9][n_c][n_e / nodes].
+hipError_t launch_bin_average(const float* P, int n_c, int n_e, int row, long long plane, int nodes,
+                              const float* weights, float* out,
                               hipStream_t s);
 
 }  // namespace cp3

@@ -189,7 +189,9 @@ int cp3h_batch_create(int device, int n_sets, const int* n_cosines, const int* n
         s.model_off = model_n;
         s.geom_off = geom_n;
         s.tab_off = tab_n;
-        s.out_off = out_n;
+        s.out_row = int(cp3::round_up(unsigned(s.n_e), cp3::kOutRowAlign));
+        s.out_plane = (long long)s.n_c * s.out_row;
+        s.out_off = out_n;                   // a multiple of kOutRowAlign
         const long long geom = (long long)s.n_c * (s.rows + 1);
         const long long cells = (long long)(s.rows + 1) * s.ne_pad;
         cos_n += s.n_c;
@@ -197,7 +199,7 @@ int cp3h_batch_create(int device, int n_sets, const int* n_cosines, const int* n
         model_n += s.rows;
         geom_n += geom;
         tab_n += cells;
-        out_n += 9LL * s.n_c * s.n_e;
+        out_n += 9LL * s.out_plane;
         b->geom_prefix[i + 1] = b->geom_prefix[i] + geom;
         b->spec_prefix[i + 1] = b->spec_prefix[i] + (s.n_e > 0 ? cells : 0);
         b->item_prefix[i + 1] = b->item_prefix[i] + (long long)((s.n_c + cp3::kRowsPerWave - 1) / cp3::kRowsPerWave) *
@@ -451,7 +453,9 @@ int cp3h_batch_copy_results_f32(cp3h_batch* b, int set, float* dst, size_t count
     if (count > 0 && !dst) return fail(CP3H_ERR_ARGUMENT, "null destination");
     if (!b->d_out) return fail(CP3H_ERR_STATE, "no unbinned results have been computed");
     if (int rc = drain(b)) return rc;
-    if (count) CP3H_HIP(hipMemcpy(dst, b->d_out + s.out_off, count * sizeof(float), hipMemcpyDeviceToHost));
+    if (count)   // 9 * n_c rows of n_e floats, device rows out_row apart
+        CP3H_HIP(hipMemcpy2D(dst, sizeof(float) * s.n_e, b->d_out + s.out_off, sizeof(float) * s.out_row,
+                             sizeof(float) * s.n_e, std::size_t(9) * s.n_c, hipMemcpyDeviceToHost));
     return CP3H_OK;
 }
 
@@ -467,8 +471,8 @@ int cp3h_batch_bin_average_f32(cp3h_batch* b, int first, int count, int nodes, c
     for (int i = first; i < first + count; i++) {
         const cp3::BatchSet& s = b->layout[i];
         if (s.n_e % nodes != 0) return fail(CP3H_ERR_ARGUMENT, "set energies are not a whole number of bins");
-        CP3H_HIP(cp3::launch_bin_average(b->d_out + s.out_off, s.n_c, s.n_e, nodes, weights, out_device + off,
-                                         b->stream));
+        CP3H_HIP(cp3::launch_bin_average(b->d_out + s.out_off, s.n_c, s.n_e, s.out_row, s.out_plane, nodes, weights,
+                                         out_device + off, b->stream));
         off += std::size_t(9) * s.n_c * (s.n_e / nodes);
     }
     CP3H_HIP(hipStreamSynchronize(b->stream));
@@ -480,6 +484,14 @@ int cp3h_batch_device_results(cp3h_batch* b, int set, const float** ptr) {
     if (!ptr) return fail(CP3H_ERR_ARGUMENT, "null output pointer");
     if (int rc = ensure_out(b)) return rc;
     *ptr = b->d_out + b->layout[set].out_off;
+    return CP3H_OK;
+}
+
+int cp3h_batch_result_strides(cp3h_batch* b, int set, long long* plane_stride, int* row_stride) {
+    if (int rc = check(b, set)) return rc;
+    if (!plane_stride || !row_stride) return fail(CP3H_ERR_ARGUMENT, "null output pointer");
+    *plane_stride = b->layout[set].out_plane;
+    *row_stride = b->layout[set].out_row;
     return CP3H_OK;
 }
 

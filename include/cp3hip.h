@@ -180,9 +180,14 @@ int cp3h_batch_compute(cp3h_batch* b);
    (pairs_ms is -1 when the computation was a graph replay). */
 int cp3h_batch_last_ms(cp3h_batch* b, float* total_ms, float* pairs_ms);
 
-/* Results of set i, SoA [t][ic][ie] float, count = 9 * n_cosines[i] * n_energies[i]. */
+/* Results of set i, SoA [t][ic][ie] float, count = 9 * n_cosines[i] * n_energies[i] (copied
+   densely).  On the device each cosine row starts a 128-byte line: P(t, ic, ie) is at
+   ptr[t * plane_stride + ic * row_stride + ie], row_stride = n_energies[i] rounded up to 32, so
+   that the kernel's row stores never share a line with another row (no counterpart in the
+   reference, whose batches do not exist). */
 int cp3h_batch_copy_results_f32(cp3h_batch* b, int set, float* dst, size_t count);
 int cp3h_batch_device_results(cp3h_batch* b, int set, const float** ptr);
+int cp3h_batch_result_strides(cp3h_batch* b, int set, long long* plane_stride, int* row_stride);
 
 /* Energy-bin averages of sets first .. first+count-1 whose energy lists are `nodes`
    quadrature nodes per bin (bin-major): out_device receives, set after set, the SoA array

@@ -208,3 +208,25 @@ def test_row_limits(graph):
         assert np.all(got[i][:, ~keep] == -5.0)
         keep_e = np.repeat(keep, 16, axis=1)
         np.testing.assert_array_equal(limited[i][:, keep_e], full[i][:, keep_e])
+
+
+def test_device_results_layout():
+    """Device results: every cosine row starts a 128-byte line (row stride = n_e rounded up to
+    32 floats); read through the reported strides they equal the dense copy."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    sets = small_sets(6, seed=31)
+    print(f"[gpu] device results layout, {len(sets)} sets", flush=True)
+    with BatchPropagator(sets, graph=False) as b:
+        b.compute()
+        for i, s in enumerate(sets):
+            plane, row = b.result_strides(i)
+            n_c, n_e = len(s.cosines), len(s.energies)
+            assert row == -(-n_e // 32) * 32 and plane == n_c * row
+            ptr = b.device_results(i)
+            assert ptr % 128 == 0
+            dense = b.results(i)
+            raw = np.empty(9 * plane, np.float32)
+            assert hip.hipMemcpy(raw.ctypes.data, ptr, raw.nbytes, 2) == 0      # device to host
+            np.testing.assert_array_equal(raw.reshape(9, n_c, row)[:, :, :n_e], dense)
