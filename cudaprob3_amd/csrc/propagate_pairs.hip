@@ -211,10 +211,10 @@ __global__ __launch_bounds__(kBlock) void pairs_mixed_kernel(MixedArgs a) {
     auto emit = [&](int ic, const float (&p)[3][3]) {
         if (ie >= a.n_e) return;
         if (kLike) {
-            const float* d = a.data + std::size_t(ic) * a.n_e + ie;
+            const float* d = a.data + std::size_t(ic) * a.n_e + ie;      // read once: non-temporal
 #pragma unroll
             for (int t = 0; t < 9; t++) {
-                const double e = double(p[t / 3][t % 3]) - double(d[t * plane]);
+                const double e = double(p[t / 3][t % 3]) - double(__builtin_nontemporal_load(d + t * plane));
                 acc = __builtin_fma(e, e, acc);
             }
         } else {
