@@ -1,11 +1,13 @@
-// One (cosZ, E) pair of the mixed-precision propagation, shared by the single-grid and the
-// batched kernels.  Layer algebra and table layout: see propagate_pairs.hip.
-//   L   : shells crossed (maxlayers of the row)
-//   gl  : layer lengths of the row in km, gl[0..L]
-//   Q, R: the energy tile's spectrum table (float4 groups / double2 phase rates in quarter turns
-//         per km), row stride st
-//   ie  : energy index (< ne_pad)
-// Writes p[in][out] = P(in -> out).
+// The (cosZ, E) pairs of the mixed-precision propagation, shared by the single-grid and the
+// batched kernels: propagate_rows_mixed<kRows> takes kRows cosine rows with the same number of
+// crossed shells at one energy per lane.  Layer algebra and table layout: see
+// propagate_pairs.hip.
+//   L    : shells crossed (maxlayers of the rows)
+//   gl[k]: layer lengths of row k in km, gl[k][0..L]
+//   Q, R : the spectrum table (float4 groups / double2 phase rates in quarter turns per km),
+//          row stride st
+//   ie   : energy index (< ne_pad)
+// Writes p[k][in][out] = P(in -> out) of row k.
 #pragma once
 #include "common.hpp"
 
@@ -250,14 +252,6 @@ __device__ inline void propagate_rows_mixed(int L, const double* const* gl, cons
         pk[2][1] = clamp01(1.0f - pk[0][1] - pk[1][1]);
         pk[2][2] = clamp01(1.0f - pk[2][0] - pk[2][1]);
     }
-}
-
-// One cosine row (the batched kernel's work item).
-__device__ inline void propagate_pair_mixed(int L, const double* __restrict__ gl, const float4* __restrict__ Q,
-                                            const double2* __restrict__ R, std::size_t st, int ie,
-                                            float p[3][3]) {
-    const double* rows[1] = {gl};
-    propagate_rows_mixed<1>(L, rows, Q, R, st, ie, reinterpret_cast<float (*)[3][3]>(p));
 }
 
 }  // namespace cp3
