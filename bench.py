@@ -230,6 +230,22 @@ def run_mcmc(args, sync):
                      "kernel_ms": k_ms, "all_kernels_ms": total_ms, "algorithmic_flops_per_launch": flops},
     }
     chain.close()
+    if not args.no_cpu_baseline:
+        # one Markov-chain step's propagation on a sample of the grid: every 16th cosine row
+        # (512 rows from -1 to 1, so the deep 100-shell paths are in proportion) x all energies
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        kind = "reference" if oracle.ref_available() else "port"
+        fn = oracle.ref_propagate if kind == "reference" else oracle.propagate
+        rows = grid.cosines[::16]
+        t0 = time.perf_counter()
+        fn(grid.antinu, rows, grid.energies, grid.radii, grid.rhos, grid.h_km, grid.params, threads=threads)
+        cdt = time.perf_counter() - t0
+        pairs = len(rows) * len(grid.energies)
+        result["cpu_baseline"] = {"value": pairs / cdt, "unit": "pairs/s", "cores": threads, "kind": kind,
+                                  "sample": f"every 16th cosine row x all {len(grid.energies)} energies "
+                                            f"({pairs} pairs, 100 shells), {cdt:.1f} s"}
+        log(f"[bench] CPU baseline {pairs / cdt:.4e} pairs/s")
     if args.binned_chain_steps > 0:
         result["binned_chains"] = binned_chain_comparison(args, sync, grid)
     return result
