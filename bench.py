@@ -290,6 +290,28 @@ def binned_chain_comparison(args, sync, grid):
     return out
 
 
+def surrogate_cpu_baseline(args):
+    """Bin averages by the reference CpuPropagator<double>: one hypothesis, every 16th cosine row
+    x all bins, 16 Gauss-Legendre nodes each (the exact method's work on the CPU)."""
+    import oracle
+    from cudaprob3_amd import surrogate as S
+    from cudaprob3_amd import workloads as W
+    threads = min(16, os.cpu_count() or 1)
+    kind = "reference" if oracle.ref_available() else "port"
+    fn = oracle.ref_propagate if kind == "reference" else oracle.propagate
+    cos = np.linspace(-1, 1, args.n)[::16]
+    E, _ = S.gl_energies(np.geomspace(0.05, 1000, args.n))
+    radii, rhos = W.jitter12_model()
+    t0 = time.perf_counter()
+    fn(0, cos, E, radii, rhos, W.PRODUCTION_HEIGHT_KM, W.NUFIT, threads=threads)
+    dt = time.perf_counter() - t0
+    bins = len(cos) * args.n
+    log(f"[bench] CPU baseline {bins / dt:.4e} bins/s")
+    return {"value": bins / dt, "unit": "bins/s", "cores": threads, "kind": kind,
+            "sample": f"one hypothesis, every 16th cosine row x {args.n} bins x 16 nodes "
+                      f"({len(cos) * len(E)} pairs), {dt:.1f} s"}
+
+
 def surrogate_comparison(args, sync):
     """Binned probabilities of many hypotheses on the 2048^2 grid: the trained network with the
     exact fallback (bins whose error-map bound exceeds 1e-3 / 1.5), against the exact
@@ -500,6 +522,7 @@ def main():
             "dtype": "bf16 MFMA (fp32 accumulation) + f32/f64 exact fallback", "data": "synthetic",
             "config": {"workload": "surrogate2048: " + sur["workload"]},
             "roofline": sur["network_roofline"], "surrogate": sur,
+            "cpu_baseline": None if args.no_cpu_baseline else surrogate_cpu_baseline(args),
             "wall_s": time.perf_counter() - t0}), flush=True)
         return
 
