@@ -368,80 +368,53 @@ __global__ __launch_bounds__(kBlock) void shell_spectrum_f64(GridDesc g, OscPara
     }
 }
 
-// Mixed-precision table, per (slot s, energy e).  The layer amplitude up to a global phase is
-// I + (e^{i dphi_1} - 1) Q_1 + (e^{i dphi_2} - 1) Q_2, since sum_k Q_k = I, with
-// dphi_k = -2.534 (lambda_k - lambda_0) L / E.  Stored: Q_1 and Q_2 in float as five float4
-// groups Qm[((s*5 + g)*ne_pad + e)*4 + c] (packed index g*4 + c: 0-8 Q_1, 9-17 Q_2, 18-19
-// zero), and dphi_k / L in quarter turns per km as Rm[(s*ne_pad + e)*2 + k - 1], double.
-//
-// Q_1 and Q_2 are formed directly in the flavour basis: with the mass-basis matrix of the eq. 11
-// products, H_m = diag(DM(n,0)) - fac conj(U_0n) U_0m (physics.hpp:361-379), the flavour matrix
-// is H = U H_m U^dagger = U diag(DM(n,0)) U^dagger - fac e_0 e_0^T (U unitary), and
-// Q_k = (H - lambda_a)(H - lambda_b) / ((lambda_k - lambda_a)(lambda_k - lambda_b)) with the
-// reference's eigenvalues and (k; a, b) pairing -- the same projectors as U P_k U^dagger, in a
-// third of the registers (the FP64 parity tables keep the reference's mass-basis route).
-__device__ inline void flavour_projectors_12(const OscParams& p, double E, double rho, double q[2][9],
-                                             double mm[3][3], double mv[3][3]) {
+// Mixed-precision table, per (slot s, energy e), in the real basis of OscParams::Hr / Wmu.
+// There the layer Hamiltonian H = Hr - fac e_0 e_0^T is real symmetric (fac as in the
+// reference, physics.hpp:277-282), so its eigenvectors v_k are real, and the layer amplitude up
+// to its global phase is
+//   S = I + (e^{i dphi_1} - 1) v_1 v_1^T + (e^{i dphi_2} - 1) v_2 v_2^T,
+// dphi_k = -2.534 (lambda_k - lambda_0) L / E (sum_k v_k v_k^T = I).  The projectors are the
+// reference's eq. 11 construction with its eigenvalues and (k; a, b) pairing
+// (physics.hpp:395-428), Q_k = (H - lambda_a)(H - lambda_b) / ((lambda_k - lambda_a)(lambda_k -
+// lambda_b)), here of a real 3x3 matrix; v_k is the column of Q_k through its largest diagonal
+// entry, normalised (Q_k = v_k v_k^T).  Stored: v_1, v_2 in float as two float4 groups
+// Qm[((s*2 + g)*ne_pad + e)*4 + c] (packed index g*4 + c: 0-2 v_1, 3-5 v_2, 6-7 zero), and
+// dphi_k / L in quarter turns per km as Rm[(s*ne_pad + e)*2 + k - 1], double.
+__device__ inline void real_eigenvectors_12(const OscParams& p, double E, double rho, double v[2][3],
+                                            double mm[3][3], double mv[3][3]) {
     matter_masses(p, E, rho, mm, mv);
     const double fac = p.antinu ? kGF * E * rho : -kGF * E * rho;
-    // H, hermitian: diagonal h_ii (real) and upper triangle h01, h02, h12
-    double hd[3];
-    Cd ho[3];
-    const int up[3][2] = {{0, 1}, {0, 2}, {1, 2}};
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        double acc = 0.0;
-#pragma unroll
-        for (int n = 1; n < 3; n++) {
-            const Cd u = Uof(p, i, n);
-            acc += p.DM[n * 3] * (u.re * u.re + u.im * u.im);
-        }
-        hd[i] = acc;
-    }
-    hd[0] -= fac;
-#pragma unroll
-    for (int w = 0; w < 3; w++) {
-        const int i = up[w][0], j = up[w][1];
-        Cd acc{0.0, 0.0};
-#pragma unroll
-        for (int n = 1; n < 3; n++) {           // U_in DM(n,0) conj(U_jn)
-            const Cd a = Uof(p, i, n), b = Uof(p, j, n);
-            acc.re += p.DM[n * 3] * (a.re * b.re + a.im * b.im);
-            acc.im += p.DM[n * 3] * (a.im * b.re - a.re * b.im);
-        }
-        ho[w] = acc;
-    }
-    auto elem = [&](int i, int j) -> Cd {      // H_ij from the stored triangle
-        if (i == j) return Cd{hd[i], 0.0};
-        const int w = (i + j) - 1;             // (0,1) -> 0, (0,2) -> 1, (1,2) -> 2
-        return i < j ? ho[w] : Cd{ho[w].re, -ho[w].im};
-    };
+    double H[3][3];
+    H[0][0] = p.Hr[0] - fac; H[1][1] = p.Hr[1]; H[2][2] = p.Hr[2];
+    H[0][1] = H[1][0] = p.Hr[3];
+    H[0][2] = H[2][0] = p.Hr[4];
+    H[1][2] = H[2][1] = p.Hr[5];
     const int pair[2][2] = {{2, 0}, {0, 1}};    // k = 1: (a, b) = (2, 0); k = 2: (0, 1)
 #pragma unroll
     for (int kk = 0; kk < 2; kk++) {
         const int k = kk + 1, a = pair[kk][0], b = pair[kk][1];
         const double la = mv[a][0], lb = mv[b][0];       // lambda - DM(0,0), DM(0,0) = 0
         const double inv = 1.0 / (mm[k][a] * mm[k][b]);
-        const int nm[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+        double Q[3][3];
 #pragma unroll
-        for (int w = 0; w < 6; w++) {
-            const int n = nm[w][0], m = nm[w][1];
-            Cd acc{0.0, 0.0};
+        for (int n = 0; n < 3; n++)
 #pragma unroll
-            for (int l = 0; l < 3; l++) {
-                Cd x = elem(n, l), y = elem(l, m);
-                if (n == l) x.re -= la;
-                if (l == m) y.re -= lb;
-                acc.re += x.re * y.re - x.im * y.im;
-                acc.im += x.re * y.im + x.im * y.re;
+            for (int m = n; m < 3; m++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int l = 0; l < 3; l++) {
+                    const double x = H[n][l] - (n == l ? la : 0.0);
+                    const double y = H[l][m] - (l == m ? lb : 0.0);
+                    acc += x * y;
+                }
+                Q[n][m] = Q[m][n] = acc * inv;
             }
-            if (w < 3) {
-                q[kk][w] = acc.re * inv;
-            } else {
-                q[kk][3 + 2 * (w - 3)] = acc.re * inv;
-                q[kk][4 + 2 * (w - 3)] = acc.im * inv;
-            }
-        }
+        int j = 0;
+        if (Q[1][1] > Q[j][j]) j = 1;
+        if (Q[2][2] > Q[j][j]) j = 2;
+        const double s = Q[j][j] > 0.0 ? 1.0 / sqrt(Q[j][j]) : 0.0;
+#pragma unroll
+        for (int n = 0; n < 3; n++) v[kk][n] = Q[n][j] * s;
     }
 }
 
@@ -449,22 +422,13 @@ __device__ inline void spectrum_cell_mixed(const GridDesc& g, const OscParams& p
                                            float* __restrict__ Qm, double* __restrict__ Rm) {
     double E, rho;
     slot_inputs(g, s, e, E, rho);
-    double q[2][9], mm[3][3], mv[3][3];
-    flavour_projectors_12(p, E, rho, q, mm, mv);
-    float packed[20];
-#pragma unroll
-    for (int f = 0; f < 9; f++) {
-        packed[f] = float(q[0][f]);
-        packed[9 + f] = float(q[1][f]);
-    }
-    packed[18] = 0.0f;
-    packed[19] = 0.0f;
+    double v[2][3], mm[3][3], mv[3][3];
+    real_eigenvectors_12(p, E, rho, v, mm, mv);
     const std::size_t st = ne_pad;
     float4* dst = reinterpret_cast<float4*>(Qm);
-#pragma unroll
-    for (int grp = 0; grp < 5; grp++)
-        dst[(std::size_t(s) * 5 + grp) * st + e] =
-            make_float4(packed[4 * grp], packed[4 * grp + 1], packed[4 * grp + 2], packed[4 * grp + 3]);
+    dst[(std::size_t(s) * kMixedQGroups + 0) * st + e] =
+        make_float4(float(v[0][0]), float(v[0][1]), float(v[0][2]), float(v[1][0]));
+    dst[(std::size_t(s) * kMixedQGroups + 1) * st + e] = make_float4(float(v[1][1]), float(v[1][2]), 0.0f, 0.0f);
     const double quarter_turn = 2.0 / (E * kPi);      // radians -> quarter turns
     Rm[(std::size_t(s) * st + e) * 2 + 0] = -kLoE * mm[1][0] * quarter_turn;
     Rm[(std::size_t(s) * st + e) * 2 + 1] = -kLoE * mm[2][0] * quarter_turn;
@@ -517,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void batch_setup(BatchDesc b) {
             const GridDesc g = batch_grid(b, set);
             const long long c = u - b.spec_prefix[k];
             spectrum_cell_mixed(g, set.p, set.ne_pad, int(c / set.ne_pad), int(c % set.ne_pad),
-                                reinterpret_cast<float*>(b.Q + set.tab_off * 5),
+                                reinterpret_cast<float*>(b.Q + set.tab_off * kMixedQGroups),
                                 reinterpret_cast<double*>(b.R + set.tab_off));
         }
     }

@@ -99,14 +99,15 @@ __device__ inline void pair_item(const BatchDesc& b, int k, long long local, int
     const bool v0 = tile * kWave < ne0, v1 = tile * kWave < ne1;
     const std::size_t gstride = set.rows + 1;
     const double* gbase = b.geom + set.geom_off;
-    const float4* Q = b.Q + set.tab_off * 5;
+    const float4* Q = b.Q + set.tab_off * kMixedQGroups;
+    const float4 wmu = make_float4(float(set.p.Wmu[0]), float(set.p.Wmu[1]), float(set.p.Wmu[2]), float(set.p.Wmu[3]));
     const double2* R = b.R + set.tab_off;
     const int L0 = v0 ? b.maxlayers[set.cos_off + ic0] : -1;
     const int L1 = v1 ? b.maxlayers[set.cos_off + ic0 + 1] : -2;
     if (L0 == L1) {
         const double* gl[2] = {gbase + std::size_t(ic0) * gstride, gbase + std::size_t(ic0 + 1) * gstride};
         float p[2][3][3];
-        propagate_rows_mixed<2>(L0, gl, Q, R, set.ne_pad, ie, p);
+        propagate_rows_mixed<2>(L0, gl, Q, R, set.ne_pad, ie, wmu, p);
         store_row<kBinned>(b, set, ic0, ne0, ie, lane, p[0]);
         store_row<kBinned>(b, set, ic0 + 1, ne1, ie, lane, p[1]);
         return;
@@ -117,7 +118,7 @@ __device__ inline void pair_item(const BatchDesc& b, int k, long long local, int
         const int ic = ic0 + r;
         const double* gl[1] = {gbase + std::size_t(ic) * gstride};
         float p[1][3][3];
-        propagate_rows_mixed<1>(r == 0 ? L0 : L1, gl, Q, R, set.ne_pad, ie, p);
+        propagate_rows_mixed<1>(r == 0 ? L0 : L1, gl, Q, R, set.ne_pad, ie, wmu, p);
         store_row<kBinned>(b, set, ic, r == 0 ? ne0 : ne1, ie, lane, p[0]);
     }
 }

@@ -23,6 +23,10 @@ struct OscParams {
     double DM[9];      // DM(i,j) = m_i^2 - m_j^2 (with the reference's degeneracy nudge)
     int order[3];      // matter root k that continues vacuum mass i
     int antinu;        // 1: matter potential sign flipped (the only nu/nubar difference)
+    // real basis of the mixed kernels (host_util.hpp real_basis): the flavour Hamiltonian is
+    // W (Hr - fac e_0 e_0^T) W^dagger with Hr real symmetric and W = diag(1, W')
+    double Hr[6];      // (H00, H11, H22, H01, H02, H12)
+    double Wmu[4];     // row 1 of W: (Re W11, Im W11, Re W12, Im W12)
 };
 
 constexpr int kWave = 64;            // CDNA wavefront
@@ -65,9 +69,11 @@ hipError_t launch_geometry(const GridDesc& g, double* geom, hipStream_t s);
 // rhos[s-1]); ne_pad = n_e rounded up to 64.
 //  FP64:  Q[(s*27 + k*9 + f)*ne_pad + e] (Q_k packed, 9 reals), R[(s*3 + k)*ne_pad + e] =
 //         -2.534 lambda_k (phase R*L/E).
-//  mixed: Qm float4 groups [(s*5 + g)*ne_pad + e] holding Q_1, Q_2 (18 floats + 2 zero),
+//  mixed: Qm float4 groups [(s*2 + g)*ne_pad + e] holding the real unit eigenvectors v_1, v_2
+//         of the layer Hamiltonian in the real basis (v_1, then v_2, then 2 zero),
 //         Rm[(s*ne_pad + e)*2 + k-1] = -2.534 (lambda_k - lambda_0) / E in quarter turns per km.
-constexpr int kMixedQFloats = 20;
+constexpr int kMixedQGroups = 2;
+constexpr int kMixedQFloats = 4 * kMixedQGroups;
 constexpr int kMixedRDoubles = 2;
 // Mixed set-up in one launch: layer lengths (geom) and the mixed spectrum table.
 hipError_t launch_setup_mixed(const GridDesc& g, const OscParams& p, int ne_pad, double* geom, float* Qm, double* Rm,
@@ -75,8 +81,9 @@ hipError_t launch_setup_mixed(const GridDesc& g, const OscParams& p, int ne_pad,
 hipError_t launch_spectrum_f64(const GridDesc& g, const OscParams& p, int ne_pad, double* Q, double* R, hipStream_t s);
 
 // Table-driven per-pair kernels.
+// wmu: row 1 of the real basis's W in float (OscParams::Wmu)
 hipError_t launch_pairs_mixed(const GridDesc& g, int ne_pad, const double* geom, const float* Q,
-                              const double* R, float* out, hipStream_t s);
+                              const double* R, float4 wmu, float* out, hipStream_t s);
 hipError_t launch_pairs_f64(const GridDesc& g, int ne_pad, const double* geom, const double* Q,
                             const double* R, double* out, hipStream_t s);
 
@@ -91,7 +98,8 @@ hipError_t launch_sum_f64(const double* in, long long n, double* scratch, hipStr
 // its pairs and nine channels, into partials[likelihood_blocks(g)]; nothing else is stored.
 unsigned likelihood_blocks(const GridDesc& g);
 hipError_t launch_pairs_mixed_sqdiff(const GridDesc& g, int ne_pad, const double* geom, const float* Q,
-                                     const double* R, const float* data, double* partials, hipStream_t s);
+                                     const double* R, float4 wmu, const float* data, double* partials,
+                                     hipStream_t s);
 // sum_i (a_i - b_i)^2 in FP64 for device floats; result in partials[reduce_partials_needed()].
 hipError_t launch_sqdiff_sum(const float* a, const float* b, std::size_t n, double* partials, hipStream_t s);
 
@@ -134,7 +142,7 @@ struct BatchDesc {
     const double* radii;
     const double* rhos;
     double* geom;
-    float4* Q;
+    float4* Q;                      // kMixedQGroups float4 per spectrum cell
     double2* R;
     unsigned long long* next_item;  // kXcds work-queue counters of the pair kernel, kQueueStride apart
                                     // (zeroed by the set-up kernel)

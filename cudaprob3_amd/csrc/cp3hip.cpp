@@ -55,6 +55,11 @@ int check_handle(cp3h_propagator* h) {
     return CP3H_OK;
 }
 
+// row mu of the mixed kernels' real basis, as the kernels read it
+float4 wmu_f32(const cp3::OscParams& p) {
+    return make_float4(float(p.Wmu[0]), float(p.Wmu[1]), float(p.Wmu[2]), float(p.Wmu[3]));
+}
+
 int sync_and_select(cp3h_propagator* h) {
     CP3H_HIP(hipSetDevice(h->device));
     CP3H_HIP(hipStreamSynchronize(h->stream));
@@ -252,6 +257,7 @@ int cp3h_set_params_f64(cp3h_propagator* h, const double* U18, const double* DM9
     std::memcpy(h->params.U, U18, sizeof(double) * 18);
     std::memcpy(h->params.DM, DM9, sizeof(double) * 9);
     vacuum_order<double>(DM9, h->params.order);
+    cp3host::real_basis(h->params.U, h->params.DM, h->params.Hr, h->params.Wmu);
     h->has_params = true;
     return CP3H_OK;
 }
@@ -262,6 +268,7 @@ int cp3h_set_params_f32(cp3h_propagator* h, const float* U18, const float* DM9) 
     for (int i = 0; i < 18; i++) h->params.U[i] = U18[i];
     for (int i = 0; i < 9; i++) h->params.DM[i] = DM9[i];
     vacuum_order<float>(DM9, h->params.order);
+    cp3host::real_basis(h->params.U, h->params.DM, h->params.Hr, h->params.Wmu);
     h->has_params = true;
     return CP3H_OK;
 }
@@ -293,7 +300,7 @@ int cp3h_compute_async(cp3h_propagator* h, int antineutrino, double production_h
                                              h->stream));
             if (h->timing) CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
             CP3H_HIP(cp3::launch_pairs_mixed(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_R,
-                                             static_cast<float*>(h->d_out), h->stream));
+                                             wmu_f32(h->params), static_cast<float*>(h->d_out), h->stream));
         } else {
             CP3H_HIP(cp3::launch_geometry(g, h->d_geom, h->stream));
             CP3H_HIP(cp3::launch_spectrum_f64(g, h->params, h->ne_pad, static_cast<double*>(h->d_Q), h->d_R, h->stream));
@@ -432,7 +439,7 @@ int cp3h_compute_sqdiff_f32(cp3h_propagator* h, int antineutrino, double product
                                      h->stream));
     if (h->timing) CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
     CP3H_HIP(cp3::launch_pairs_mixed_sqdiff(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_R,
-                                            data_device, h->d_like, h->stream));
+                                            wmu_f32(h->params), data_device, h->d_like, h->stream));
     if (h->timing) CP3H_HIP(hipEventRecord(h->ev_stop, h->stream));
     h->has_timing = h->timing;
     CP3H_HIP(cp3::launch_sum_f64(h->d_like, blocks, h->d_partials, h->stream));

@@ -227,7 +227,7 @@ int cp3h_batch_create(int device, int n_sets, const int* n_cosines, const int* n
         CP3H_HIP(hipMalloc(&b->d_radii, sizeof(double) * model_n));
         CP3H_HIP(hipMalloc(&b->d_rhos, sizeof(double) * model_n));
         CP3H_HIP(hipMalloc(&b->d_geom, sizeof(double) * std::max(geom_n, 1LL)));
-        CP3H_HIP(hipMalloc(&b->d_Q, sizeof(float4) * 5 * std::max(tab_n, 1LL)));
+        CP3H_HIP(hipMalloc(&b->d_Q, sizeof(float4) * cp3::kMixedQGroups * std::max(tab_n, 1LL)));
         CP3H_HIP(hipMalloc(&b->d_R, sizeof(double2) * std::max(tab_n, 1LL)));
         CP3H_HIP(hipMalloc(&b->d_next, sizeof(unsigned long long) * cp3::kXcds * cp3::kQueueStride));
         b->cos_total = cos_n;
@@ -317,6 +317,7 @@ int cp3h_batch_set_params_f64(cp3h_batch* b, int set, const double* U18, const d
     std::memcpy(s.p.U, U18, sizeof(double) * 18);
     std::memcpy(s.p.DM, DM9, sizeof(double) * 9);
     cp3host::vacuum_order<double>(DM9, s.p.order);
+    cp3host::real_basis(s.p.U, s.p.DM, s.p.Hr, s.p.Wmu);
     s.p.antinu = antineutrino;
     s.h_cm = production_height_cm;
     b->has_params[set] = 1;

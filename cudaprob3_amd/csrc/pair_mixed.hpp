@@ -62,147 +62,119 @@ __device__ inline void phase_factor_m1(double t, float& cm1, float& sn) {
     }
 }
 
-// One shell's table entries at this lane's energy: Q_1, Q_2 (packed, see propagate_pairs.hip)
-// and the two phase rates.  They depend on (shell, energy) only, so rows that cross the same
-// shells share them.
+// One shell's table entries at this lane's energy: the real eigenvectors v_1, v_2 and the two
+// phase rates.  They depend on (shell, energy) only, so rows that cross the same shells share
+// them.
 struct LayerTable {
-    float q[20];
+    float v[2][3];
     double2 r;
 };
 
 __device__ inline __attribute__((always_inline)) void load_table(const float4* __restrict__ qp,
                                                                 const double2* __restrict__ rp, std::size_t st,
                                                                 int i, LayerTable& t) {
-#pragma unroll
-    for (int grp = 0; grp < 5; grp++) {
-        const float4 v = qp[(std::size_t(i) * 5 + grp) * st];
-        t.q[4 * grp] = v.x; t.q[4 * grp + 1] = v.y; t.q[4 * grp + 2] = v.z; t.q[4 * grp + 3] = v.w;
-    }
+    const float4 a = qp[(std::size_t(i) * kMixedQGroups + 0) * st];
+    const float4 b = qp[(std::size_t(i) * kMixedQGroups + 1) * st];
+    t.v[0][0] = a.x; t.v[0][1] = a.y; t.v[0][2] = a.z;
+    t.v[1][0] = a.w; t.v[1][1] = b.x; t.v[1][2] = b.y;
     t.r = rp[std::size_t(i) * st];
 }
 
-// The layer amplitude up to its global phase, A = I + w_1 Q_1 + w_2 Q_2 with
-// w_k = e^{i dphi_k} - 1; Q Hermitian, so A_ji uses conj(Q_ij).
-__device__ inline __attribute__((always_inline)) void layer_amplitude_mixed(const LayerTable& t, double dist,
-                                                                           Cx<float> A[3][3]) {
-    const float* q = t.q;
-    float wr[2], wi[2];
-    phase_factor_m1(t.r.x * dist, wr[0], wi[0]);
-    phase_factor_m1(t.r.y * dist, wr[1], wi[1]);
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        A[d][d].re = __builtin_fmaf(wr[1], q[9 + d], __builtin_fmaf(wr[0], q[d], 1.0f));
-        A[d][d].im = __builtin_fmaf(wi[1], q[9 + d], wi[0] * q[d]);
-    }
-    const int idx[3][2] = {{0, 1}, {0, 2}, {1, 2}};
-#pragma unroll
-    for (int w = 0; w < 3; w++) {
-        const float a0 = q[3 + 2 * w], b0 = q[4 + 2 * w], a1 = q[12 + 2 * w], b1 = q[13 + 2 * w];
-        // w_k (a + i b) and w_k (a - i b), summed over k
-        float ure = wr[0] * a0, uim = wr[0] * b0, lre = ure, lim = wi[0] * a0;
-        ure = __builtin_fmaf(-wi[0], b0, ure);
-        uim = __builtin_fmaf(wi[0], a0, uim);
-        lre = __builtin_fmaf(wi[0], b0, lre);
-        lim = __builtin_fmaf(-wr[0], b0, lim);
-        ure = __builtin_fmaf(wr[1], a1, ure);
-        ure = __builtin_fmaf(-wi[1], b1, ure);
-        uim = __builtin_fmaf(wr[1], b1, uim);
-        uim = __builtin_fmaf(wi[1], a1, uim);
-        lre = __builtin_fmaf(wr[1], a1, lre);
-        lre = __builtin_fmaf(wi[1], b1, lre);
-        lim = __builtin_fmaf(wi[1], a1, lim);
-        lim = __builtin_fmaf(-wr[1], b1, lim);
-        A[idx[w][0]][idx[w][1]] = Cx<float>{ure, uim};
-        A[idx[w][1]][idx[w][0]] = Cx<float>{lre, lim};
-    }
+// w_k = e^{i dphi_k} - 1 of a layer of length dist km
+struct LayerPhases {
+    float re[2], im[2];
+};
+
+__device__ inline __attribute__((always_inline)) LayerPhases layer_phases(const LayerTable& t, double dist) {
+    LayerPhases w;
+    phase_factor_m1(t.r.x * dist, w.re[0], w.im[0]);
+    phase_factor_m1(t.r.y * dist, w.re[1], w.im[1]);
+    return w;
 }
 
-// F <- A F (F: columns 0-1 of the forward product)
-__device__ inline __attribute__((always_inline)) void left_mul_cols(const Cx<float> A[3][3], Cx<float> F[3][2]) {
-    Cx<float> G[3][2];
+// X <- S X for the layer amplitude S = I + w_1 v_1 v_1^T + w_2 v_2 v_2^T and a 3 x kCols
+// complex X, as two rank-1 updates X += v_k (w_k (v_k^T X)) (applied one after the other:
+// v_2^T v_1 = 0, so the second sees the same v_2^T X).  16 operations per column and k.
+template <int kCols>
+__device__ inline __attribute__((always_inline)) void apply_layer(const LayerTable& t, const LayerPhases& w,
+                                                                  Cx<float> X[3][kCols]) {
+#pragma clang fp contract(off)
 #pragma unroll
-    for (int x = 0; x < 3; x++)
+    for (int k = 0; k < 2; k++) {
+        const float* v = t.v[k];
 #pragma unroll
-        for (int y = 0; y < 2; y++) {
-            Cx<float> acc = cmul(A[x][0], F[0][y]);
-            cmac(acc, A[x][1], F[1][y]);
-            cmac(acc, A[x][2], F[2][y]);
-            G[x][y] = acc;
+        for (int j = 0; j < kCols; j++) {
+            float cr = v[0] * X[0][j].re, ci = v[0] * X[0][j].im;
+            cr = __builtin_fmaf(v[1], X[1][j].re, cr);
+            ci = __builtin_fmaf(v[1], X[1][j].im, ci);
+            cr = __builtin_fmaf(v[2], X[2][j].re, cr);
+            ci = __builtin_fmaf(v[2], X[2][j].im, ci);
+            float dr = w.re[k] * cr, di = w.re[k] * ci;
+            dr = __builtin_fmaf(-w.im[k], ci, dr);
+            di = __builtin_fmaf(w.im[k], cr, di);
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                X[a][j].re = __builtin_fmaf(v[a], dr, X[a][j].re);
+                X[a][j].im = __builtin_fmaf(v[a], di, X[a][j].im);
+            }
         }
-#pragma unroll
-    for (int x = 0; x < 3; x++)
-#pragma unroll
-        for (int y = 0; y < 2; y++) F[x][y] = G[x][y];
+    }
 }
 
-// M <- M A (M: rows 0-1 of the outgoing mantle product)
-__device__ inline __attribute__((always_inline)) void right_mul_rows(Cx<float> M[2][3], const Cx<float> A[3][3]) {
-    Cx<float> N[2][3];
+// Y = S (the first shell): Y_ab = delta_ab + sum_k w_k v_ka v_kb, complex symmetric.
+__device__ inline __attribute__((always_inline)) void init_layer(const LayerTable& t, const LayerPhases& w,
+                                                                 Cx<float> Y[3][3]) {
+#pragma clang fp contract(off)
 #pragma unroll
-    for (int x = 0; x < 2; x++)
+    for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int y = 0; y < 3; y++) {
-            Cx<float> acc = cmul(M[x][0], A[0][y]);
-            cmac(acc, M[x][1], A[1][y]);
-            cmac(acc, M[x][2], A[2][y]);
-            N[x][y] = acc;
+        for (int b = a; b < 3; b++) {
+            const float p0 = t.v[0][a] * t.v[0][b], p1 = t.v[1][a] * t.v[1][b];
+            Cx<float> y;
+            y.re = __builtin_fmaf(w.re[1], p1, __builtin_fmaf(w.re[0], p0, a == b ? 1.0f : 0.0f));
+            y.im = __builtin_fmaf(w.im[1], p1, w.im[0] * p0);
+            Y[a][b] = y;
+            Y[b][a] = y;
         }
-#pragma unroll
-    for (int x = 0; x < 2; x++)
-#pragma unroll
-        for (int y = 0; y < 3; y++) M[x][y] = N[x][y];
 }
 
-// The path's layers in the reference's order (physics.hpp:645-677): layer 0 (atmosphere), then
-// shells 1..L-1 on the way in, which enter both the forward product and the mantle product
-// of the way out, then the innermost shell L, crossed once; total = mantle x forward.  The
-// first two layers and the last are peeled off the loop, so its body has no branches; L is
+// The path's layers (physics.hpp:645-677): layer 0 (atmosphere), shells 1..L-1 on the way in,
+// the innermost shell L, shells L-1..1 on the way out; total T = T_1..T_{L-1} T_L T_{L-1}..T_1 T_0.
+// In the real basis every layer amplitude is W S_i W^dagger with S_i complex symmetric, so with
+// Y = S_{L-1} ... S_1 (one running product over the shells of the way in) the way out is
+// S_1 ... S_{L-1} = Y^T, and
+//   T = W Y^T S_L Y S_0 W^dagger.
+// Only the flavour rows and columns e, mu of T are needed (unitarity gives the rest): with
+// G = S_0 W^dagger[:, e mu] (3 x 2) and the rows e, mu of W Y^T, the 2 x 2 block is
+// (W Y^T)[e mu, :] S_L (Y G).  Per shell of the way in one 3x3 update (two rank-1 terms); the
+// atmosphere, the innermost shell and the rotation are applied once per pair.  L is
 // wave-uniform, so every branch here is a scalar one.
 //
 // kRows cosine rows with the same L are propagated together at this lane's energy: they cross
 // the same shells, so each shell's table entries are loaded once for all of them (only the
 // layer lengths, hence the phases, differ), which halves the table reads per pair for kRows = 2.
+// wmu = row mu of W (OscParams::Wmu); row e of W is e_0^T.
 template <int kRows>
 __device__ inline void propagate_rows_mixed(int L, const double* const* gl, const float4* __restrict__ Q,
-                                            const double2* __restrict__ R, std::size_t st, int ie,
+                                            const double2* __restrict__ R, std::size_t st, int ie, float4 wmu,
                                             float (*p)[3][3]) {
+#pragma clang fp contract(off)
     const float4* qp = Q + ie;
     const double2* rp = R + ie;
     LayerTable t;
-    Cx<float> A[kRows][3][3];
-    Cx<float> F[kRows][3][2];   // columns 0-1 of T_i ... T_1 T_atm
-    Cx<float> M[kRows][2][3];   // rows 0-1 of T_1 ... T_i
-    load_table(qp, rp, st, 0, t);
-#pragma unroll
-    for (int k = 0; k < kRows; k++) {
-        layer_amplitude_mixed(t, gl[k][0], A[k]);
-#pragma unroll
-        for (int x = 0; x < 3; x++)
-#pragma unroll
-            for (int y = 0; y < 2; y++) F[k][x][y] = A[k][x][y];
-    }
+    Cx<float> Y[kRows][3][3];   // S_{i} ... S_1
+    Cx<float> G[kRows][3][2];   // columns e, mu of ... S_0 W^dagger
     if (L >= 2) {
         load_table(qp, rp, st, 1, t);
 #pragma unroll
-        for (int k = 0; k < kRows; k++) {
-            layer_amplitude_mixed(t, gl[k][1], A[k]);
-            left_mul_cols(A[k], F[k]);
-#pragma unroll
-            for (int x = 0; x < 2; x++)
-#pragma unroll
-                for (int y = 0; y < 3; y++) M[k][x][y] = A[k][x][y];
-        }
-        // one row: unrolled twice, so that the running products alternate registers instead of
-        // being copied; two rows carry enough independent work as they are
+        for (int k = 0; k < kRows; k++) init_layer(t, layer_phases(t, gl[k][1]), Y[k]);
         auto shell = [&](int i) {
             load_table(qp, rp, st, i, t);
 #pragma unroll
-            for (int k = 0; k < kRows; k++) {
-                layer_amplitude_mixed(t, gl[k][i], A[k]);
-                left_mul_cols(A[k], F[k]);
-                right_mul_rows(M[k], A[k]);
-            }
+            for (int k = 0; k < kRows; k++) apply_layer<3>(t, layer_phases(t, gl[k][i]), Y[k]);
         };
+        // one row: unrolled twice for independent work across shells; two rows carry enough
+        // independent work as they are
         if (kRows == 1) {
 #pragma unroll 2
             for (int i = 2; i < L; i++) shell(i);
@@ -211,41 +183,96 @@ __device__ inline void propagate_rows_mixed(int L, const double* const* gl, cons
             for (int i = 2; i < L; i++) shell(i);
         }
     }
+    // atmosphere: G = S_0 W^dagger[:, (e, mu)] = W^dagger[:, (e, mu)] + sum_k v_k w_k z_k^T,
+    // z_k = W[(e, mu), :] v_k = (v_k0, W_mu1 v_k1 + W_mu2 v_k2) conjugated
+    load_table(qp, rp, st, 0, t);
+#pragma unroll
+    for (int k = 0; k < kRows; k++) {
+        const LayerPhases w = layer_phases(t, gl[k][0]);
+        Cx<float>(&g)[3][2] = G[k];
+        g[0][0] = Cx<float>{1.0f, 0.0f}; g[1][0] = Cx<float>{0.0f, 0.0f}; g[2][0] = Cx<float>{0.0f, 0.0f};
+        g[0][1] = Cx<float>{0.0f, 0.0f}; g[1][1] = Cx<float>{wmu.x, -wmu.y}; g[2][1] = Cx<float>{wmu.z, -wmu.w};
+#pragma unroll
+        for (int kk = 0; kk < 2; kk++) {
+            const float* v = t.v[kk];
+            const float zr = __builtin_fmaf(wmu.z, v[2], wmu.x * v[1]);
+            const float zi = -__builtin_fmaf(wmu.w, v[2], wmu.y * v[1]);
+            const float er = w.re[kk] * v[0], ei = w.im[kk] * v[0];
+            float mr = w.re[kk] * zr, mi = w.re[kk] * zi;
+            mr = __builtin_fmaf(-w.im[kk], zi, mr);
+            mi = __builtin_fmaf(w.im[kk], zr, mi);
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                g[a][0].re = __builtin_fmaf(v[a], er, g[a][0].re);
+                g[a][0].im = __builtin_fmaf(v[a], ei, g[a][0].im);
+                g[a][1].re = __builtin_fmaf(v[a], mr, g[a][1].re);
+                g[a][1].im = __builtin_fmaf(v[a], mi, g[a][1].im);
+            }
+        }
+    }
+    if (L >= 2) {
+#pragma unroll
+        for (int k = 0; k < kRows; k++) {           // G <- Y G
+            Cx<float> N[3][2];
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    Cx<float> acc = cmul(Y[k][a][0], G[k][0][b]);
+                    cmac(acc, Y[k][a][1], G[k][1][b]);
+                    cmac(acc, Y[k][a][2], G[k][2][b]);
+                    N[a][b] = acc;
+                }
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) G[k][a][b] = N[a][b];
+        }
+    }
     if (L >= 1) {
         load_table(qp, rp, st, L, t);
 #pragma unroll
-        for (int k = 0; k < kRows; k++) {
-            layer_amplitude_mixed(t, gl[k][L], A[k]);
-            left_mul_cols(A[k], F[k]);
-        }
+        for (int k = 0; k < kRows; k++) apply_layer<2>(t, layer_phases(t, gl[k][L]), G[k]);
     }
+    const Cx<float> w1{wmu.x, wmu.y}, w2{wmu.z, wmu.w};
 #pragma unroll
     for (int k = 0; k < kRows; k++) {
-        // upper-left 2x2 block S[out][in] of the total amplitude
-        Cx<float> S[2][2];
+        // T[out][in] for out, in in (e, mu): rows e, mu of W Y^T (W for L < 2) times G
+        Cx<float> T[2][2];
         if (L >= 2) {
+            Cx<float> He[3], Hm[3];
 #pragma unroll
-            for (int x = 0; x < 2; x++)
+            for (int c = 0; c < 3; c++) {
+                He[c] = Y[k][c][0];
+                Hm[c] = cmul(w1, Y[k][c][1]);
+                cmac(Hm[c], w2, Y[k][c][2]);
+            }
 #pragma unroll
-                for (int y = 0; y < 2; y++) {
-                    Cx<float> acc = cmul(M[k][x][0], F[k][0][y]);
-                    cmac(acc, M[k][x][1], F[k][1][y]);
-                    cmac(acc, M[k][x][2], F[k][2][y]);
-                    S[x][y] = acc;
-                }
+            for (int b = 0; b < 2; b++) {
+                Cx<float> e = cmul(He[0], G[k][0][b]), m = cmul(Hm[0], G[k][0][b]);
+                cmac(e, He[1], G[k][1][b]);
+                cmac(m, Hm[1], G[k][1][b]);
+                cmac(e, He[2], G[k][2][b]);
+                cmac(m, Hm[2], G[k][2][b]);
+                T[0][b] = e;
+                T[1][b] = m;
+            }
         } else {
 #pragma unroll
-            for (int x = 0; x < 2; x++)
-#pragma unroll
-                for (int y = 0; y < 2; y++) S[x][y] = F[k][x][y];
+            for (int b = 0; b < 2; b++) {
+                T[0][b] = G[k][0][b];
+                Cx<float> m = cmul(w1, G[k][1][b]);
+                cmac(m, w2, G[k][2][b]);
+                T[1][b] = m;
+            }
         }
-        // p[in][out] = |S[out][in]|^2; the third row and column from unitarity
+        // p[in][out] = |T[out][in]|^2; the third row and column from unitarity
         float (&pk)[3][3] = p[k];
 #pragma unroll
         for (int in = 0; in < 2; in++)
 #pragma unroll
             for (int o = 0; o < 2; o++)
-                pk[in][o] = __builtin_fmaf(S[o][in].re, S[o][in].re, S[o][in].im * S[o][in].im);
+                pk[in][o] = __builtin_fmaf(T[o][in].re, T[o][in].re, T[o][in].im * T[o][in].im);
         pk[0][2] = clamp01(1.0f - pk[0][0] - pk[0][1]);
         pk[1][2] = clamp01(1.0f - pk[1][0] - pk[1][1]);
         pk[2][0] = clamp01(1.0f - pk[0][0] - pk[1][0]);

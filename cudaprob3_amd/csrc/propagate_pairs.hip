@@ -11,18 +11,22 @@
 // expression ((-2.534 lambda_k) L) / E, and the reference's running products of full 3x3
 // matrices.
 //
-// Mixed precision (pairs_mixed): three changes that are exact in real arithmetic.
+// Mixed precision (pairs_mixed): changes that are exact in real arithmetic.
+//  * A real basis.  The flavour Hamiltonian of every layer is W H_r W^dagger with H_r real
+//    symmetric and one unitary W per hypothesis (W = diag(1, W'), it commutes with the
+//    electron-only matter term; host_util.hpp real_basis).  So the layer amplitudes are
+//    W S W^dagger with S = sum_k e^{i phi_k} v_k v_k^T for real eigenvectors v_k: complex
+//    symmetric.  The mantle product of the way out is then the transpose of the product of the
+//    way in, and one running product serves both (pair_mixed.hpp).
 //  * The layer's global phase e^{i phi_0} is dropped (it cancels in |amplitude|^2), and
-//    sum_k Q_k = I gives A' = I + (e^{i dphi_1} - 1) Q_1 + (e^{i dphi_2} - 1) Q_2 with
-//    dphi_k = phi_k - phi_0.  dphi_k (up to ~10^3 rad on core-crossing paths at 0.05 GeV,
-//    where FP32 would lose 1e-4 rad) is formed and reduced to [-pi/4, pi/4] plus a quadrant
-//    in FP64; sin/cos of the remainder (polynomials, pair_mixed.hpp) and the matrix
-//    arithmetic are FP32.
+//    sum_k v_k v_k^T = I gives S' = I + (e^{i dphi_1} - 1) v_1 v_1^T + (e^{i dphi_2} - 1) v_2 v_2^T
+//    with dphi_k = phi_k - phi_0: two real rank-1 updates per layer.  dphi_k (up to ~10^3 rad
+//    on core-crossing paths at 0.05 GeV, where FP32 would lose 1e-4 rad) is formed and reduced
+//    to [-pi/4, pi/4] plus a quadrant in FP64; sin/cos of the remainder (polynomials,
+//    pair_mixed.hpp) and the matrix arithmetic are FP32.
 //  * The total amplitude is unitary, so its upper-left 2x2 block fixes all nine
-//    probabilities (rows and columns of |S|^2 sum to one).  Only columns 0-1 of the forward
-//    product and rows 0-1 of the mantle product are carried: 18 + 18 complex MACs per
-//    mantle layer instead of 27 + 27.
-//  * Q_1, Q_2 are read as five coalesced float4 loads per lane and layer.
+//    probabilities (rows and columns of |T|^2 sum to one).
+//  * v_1, v_2 are read as two coalesced float4 loads per lane and layer.
 //
 // Mapping: one wavefront = two cosine rows x 64 consecutive energies (lane = energy); the four
 // wavefronts of a workgroup take eight consecutive rows of the same energy tile, and
@@ -176,8 +180,9 @@ __global__ __launch_bounds__(kBlock) void pairs_f64_kernel(PairArgs a) {
 struct MixedArgs {
     const int* maxlayers;
     const double* geom;       // [n_c][rows + 1] km
-    const float4* Q;          // [(rows + 1) * 5][ne_pad]
+    const float4* Q;          // [(rows + 1) * kMixedQGroups][ne_pad]
     const double2* R;         // [(rows + 1)][ne_pad]
+    float4 wmu;               // row mu of the real basis's W
     float* out;               // SoA [9][n_c][n_e]
     const float* data;        // likelihood variant: SoA [9][n_c][n_e] reference values
     double* partials;         // likelihood variant: one sum per workgroup
@@ -229,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void pairs_mixed_kernel(MixedArgs a) {
     if (n_rows == 2 && a.maxlayers[ic0 + 1] == L0) {
         const double* gl[2] = {a.geom + std::size_t(ic0) * (a.rows + 1), a.geom + std::size_t(ic0 + 1) * (a.rows + 1)};
         float p[2][3][3];
-        propagate_rows_mixed<2>(L0, gl, a.Q, a.R, st, ie, p);
+        propagate_rows_mixed<2>(L0, gl, a.Q, a.R, st, ie, a.wmu, p);
         emit(ic0, p[0]);
         emit(ic0 + 1, p[1]);
     } else {
@@ -238,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void pairs_mixed_kernel(MixedArgs a) {
             const int ic = ic0 + k;
             const double* gl[1] = {a.geom + std::size_t(ic) * (a.rows + 1)};
             float p[1][3][3];
-            propagate_rows_mixed<1>(a.maxlayers[ic], gl, a.Q, a.R, st, ie, p);
+            propagate_rows_mixed<1>(a.maxlayers[ic], gl, a.Q, a.R, st, ie, a.wmu, p);
             emit(ic, p[0]);
         }
     }
@@ -269,19 +274,20 @@ unsigned likelihood_blocks(const GridDesc& g) {
 }
 
 hipError_t launch_pairs_mixed(const GridDesc& g, int ne_pad, const double* geom, const float* Q,
-                              const double* R, float* out, hipStream_t s) {
+                              const double* R, float4 wmu, float* out, hipStream_t s) {
     if (g.n_c == 0 || g.n_e == 0) return hipSuccess;
-    MixedArgs a{g.maxlayers, geom, reinterpret_cast<const float4*>(Q), reinterpret_cast<const double2*>(R), out,
-                nullptr, nullptr, g.n_c, g.n_e, g.rows, ne_pad};
+    MixedArgs a{g.maxlayers, geom, reinterpret_cast<const float4*>(Q), reinterpret_cast<const double2*>(R), wmu,
+                out, nullptr, nullptr, g.n_c, g.n_e, g.rows, ne_pad};
     hipLaunchKernelGGL(pairs_mixed_kernel<false>, dim3(likelihood_blocks(g)), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_pairs_mixed_sqdiff(const GridDesc& g, int ne_pad, const double* geom, const float* Q,
-                                     const double* R, const float* data, double* partials, hipStream_t s) {
+                                     const double* R, float4 wmu, const float* data, double* partials,
+                                     hipStream_t s) {
     if (g.n_c == 0 || g.n_e == 0) return hipSuccess;
-    MixedArgs a{g.maxlayers, geom, reinterpret_cast<const float4*>(Q), reinterpret_cast<const double2*>(R), nullptr,
-                data, partials, g.n_c, g.n_e, g.rows, ne_pad};
+    MixedArgs a{g.maxlayers, geom, reinterpret_cast<const float4*>(Q), reinterpret_cast<const double2*>(R), wmu,
+                nullptr, data, partials, g.n_c, g.n_e, g.rows, ne_pad};
     hipLaunchKernelGGL(pairs_mixed_kernel<true>, dim3(likelihood_blocks(g)), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }

@@ -2,15 +2,19 @@
 reference's golden vectors: pins the precision design itself, independently of a GPU.
 
 What is emulated, stage by stage (numpy, vectorised over the energies of a cosine row):
-  * the spectrum per (shell, energy) in FP64 -- eigenvalues and flavour-basis projectors
-    Q_k = U P_k U^dagger of the layer Hamiltonian, here from numpy.linalg.eigh instead of the
-    reference's cubic formula (equal to ~1e-16), Q_1 and Q_2 rounded to float32 as stored;
+  * the real basis (host_util.hpp real_basis): the unitary W = diag(1, W') that makes the vacuum
+    flavour Hamiltonian U diag(0, DM(1,0), DM(2,0)) U^dagger real symmetric (Hr) while fixing
+    the electron flavour, built from U as the library builds it;
+  * the spectrum per (shell, energy) in FP64 -- eigenvalues and real unit eigenvectors v_k of
+    Hr + a e_0 e_0^T, here from numpy.linalg.eigh instead of the reference's cubic formula and
+    eq. 11 projectors (equal to ~1e-16), v_1 and v_2 rounded to float32 as stored;
   * the phase differences dphi_k = -2.534 (lambda_k - lambda_0) L / E in quarter turns, reduced in
     FP64 (q = rint(t), r = t - q), the remainder z = r pi / 2 rounded to float32, sin z and
     cos z - 1 by the kernel's Taylor polynomials in float32, rotated by the quadrant q mod 4;
-  * A = I + w_1 Q_1 + w_2 Q_2, the running products (columns 0-1 forward, rows 0-1 mantle) and
-    S = mantle x forward in complex64; the upper-left 2x2 block of |S|^2 and the other five
-    probabilities from unitarity, clamped to [0, 1].
+  * the running product Y = S_{L-1} ... S_1 of the way in by rank-1 updates
+    Y += v_k (w_k v_k^T Y), the way out as Y^T, the atmosphere as S_0 W^dagger[:, e mu] and the
+    2x2 block (W Y^T)[e mu, :] S_L Y S_0 W^dagger[:, e mu] in complex64; the upper-left 2x2 block
+    of |T|^2 and the other five probabilities from unitarity, clamped to [0, 1].
 The bar is the mixed kernels' 1e-5 (max |dP| over all nine channels).
 """
 import numpy as np
@@ -25,16 +29,32 @@ KGF, LOE = 1.52588e-4, 2.534
 F32 = np.float32
 
 
-def spectrum(U18, dm9, E, rho_eff, antinu):
-    """Eigenvalues (ascending) and flavour-basis projectors of the layer Hamiltonian at energies E
-    (FP64): mass-basis H = diag(0, DM(1,0), DM(2,0)) + a w w^dagger, w = conj(U_e.), a = +-GF E rho."""
-    U = (U18[0::2] + 1j * U18[1::2]).reshape(3, 3)
+def real_basis(U, dm9):
+    """(Hr, W): W = diag(1, W') unitary with W^dagger M W real symmetric for the vacuum part
+    M = U diag(0, DM(1,0), DM(2,0)) U^dagger, W' = [a, b]: a along (M_10, M_20), b orthogonal to
+    it with the phase that makes (W^dagger M W)_12 real."""
+    M = sum(dm9[3 * n] * np.outer(U[:, n], np.conj(U[:, n])) for n in (1, 2))
+    x = M[1:, 0]
+    nx = np.linalg.norm(x)
+    a = x / nx if nx > 0 else np.array([1.0, 0.0], complex)
+    b = np.array([-np.conj(a[1]), np.conj(a[0])])
+    m = np.conj(a) @ M[1:, 1:] @ b
+    if abs(m) > 0:
+        b = b * np.conj(m) / abs(m)
+    W = np.eye(3, dtype=complex)
+    W[1:, 1] = a
+    W[1:, 2] = b
+    H = np.conj(W.T) @ M @ W
+    return H, W
+
+
+def spectrum(Hr, E, rho_eff, antinu):
+    """Eigenvalues (ascending) and real eigenvectors [e][component][k] of Hr + a e_0 e_0^T at
+    energies E (FP64), a = +-GF E rho."""
     a = (-1.0 if antinu else 1.0) * KGF * E * rho_eff
-    w = np.conj(U[0, :])
-    H = np.diag([0.0, dm9[3], dm9[6]]).astype(complex)[None] + a[:, None, None] * np.outer(w, np.conj(w))[None]
-    lam, V = np.linalg.eigh(H)
-    Q = np.einsum("ij,ejk,elk,ml->ekim", U, V, np.conj(V), np.conj(U))   # [e][k] U v_k v_k^dagger U^dagger
-    return lam, Q
+    H = np.repeat(Hr.real[None], len(E), axis=0)
+    H[:, 0, 0] += a
+    return np.linalg.eigh(H)
 
 
 def phase_factor_m1(t):
@@ -58,7 +78,16 @@ def phase_factor_m1(t):
     return (cm1.astype(F32) + 1j * sn.astype(F32)).astype(np.complex64)
 
 
-def emulate(g, phase=phase_factor_m1):
+def apply_layer(X, v, w):
+    """X <- S X for S = I + sum_k w_k v_k v_k^T: X [e][3][cols] complex64, v [k][e][3] float32."""
+    for k in range(2):
+        c = np.einsum("ea,eaj->ej", v[k], X).astype(np.complex64)
+        d = (w[k][:, None] * c).astype(np.complex64)
+        X = (X + v[k][:, :, None] * d[:, None, :]).astype(np.complex64)
+    return X
+
+
+def emulate(g, phase=phase_factor_m1, way_out=lambda Y: np.transpose(Y, (0, 2, 1))):
     p = Propagator(len(g["cosines"]), len(g["energies"]))
     p.setEnergyList(g["energies"])
     p.setCosineList(g["cosines"])
@@ -69,32 +98,39 @@ def emulate(g, phase=phase_factor_m1):
     antinu = int(g["antinu"])
     E = np.asarray(p.energyList, np.float64)
     n_e = len(E)
-    # per shell slot: spectrum at every energy (slot 0 = the vacuum atmosphere)
+    U18 = np.asarray(p.Mix_U)
+    Hr, W = real_basis((U18[0::2] + 1j * U18[1::2]).reshape(3, 3), np.asarray(p.dm))
+    Wc = np.conj(W[:2, :].T).astype(np.complex64)           # W^dagger[:, (e, mu)]
+    Wr = W[:2, :].astype(np.complex64)                      # W[(e, mu), :]
+    # per shell slot: phase rates and eigenvectors at every energy (slot 0 = the vacuum atmosphere)
     tables = []
     for slot in range(len(p.radii) + 1):
         rho = 0.0 if slot == 0 else float(p.rhos[slot - 1]) * 0.5
-        lam, Q = spectrum(np.asarray(p.Mix_U), np.asarray(p.dm), E, np.full(n_e, rho), antinu)
+        lam, V = spectrum(Hr, E, np.full(n_e, rho), antinu)
         rate = [(-LOE * (lam[:, k] - lam[:, 0]) / E) / (np.pi / 2) for k in (1, 2)]   # quarter turns per km
-        tables.append((rate, Q[:, 1].astype(np.complex64), Q[:, 2].astype(np.complex64)))
+        tables.append((rate, [V[:, :, k].astype(F32) for k in (1, 2)]))
     out = np.zeros((len(p.cosineList), n_e, 9))
-    eye = np.eye(3, dtype=np.complex64)[None]
     for ic, cz in enumerate(p.cosineList):
         L = int(p.maxlayers[ic])
         dist = oracle.layer_distances(p.radii, L, cz, float(p.ProductionHeightinCentimeter)) / 1e5
-        F = M = None
-        for i in range(L + 1):
-            rate, Q1, Q2 = tables[i]
-            w1 = phase(rate[0] * dist[i])
-            w2 = phase(rate[1] * dist[i])
-            A = (eye + w1[:, None, None] * Q1 + w2[:, None, None] * Q2).astype(np.complex64)
-            if i == 0:
-                F = A[:, :, :2]
-            else:
-                F = (A @ F).astype(np.complex64)
-                if i < L:
-                    M = A[:, :2, :] if i == 1 else (M @ A).astype(np.complex64)
-        S = (M @ F).astype(np.complex64) if L >= 2 else F[:, :2, :]
-        P2 = (np.abs(S) ** 2).astype(F32)                      # P2[e, out, in]
+
+        def layer(i):
+            rate, v = tables[i]
+            return v, [phase(rate[0] * dist[i]), phase(rate[1] * dist[i])]
+
+        if L >= 2:
+            v, w = layer(1)
+            Y = apply_layer(np.repeat(np.eye(3, dtype=np.complex64)[None], n_e, axis=0), v, w)
+            for i in range(2, L):
+                Y = apply_layer(Y, *layer(i))
+        G = apply_layer(np.repeat(Wc[None], n_e, axis=0), *layer(0))
+        if L >= 2:
+            G = (Y @ G).astype(np.complex64)
+        if L >= 1:
+            G = apply_layer(G, *layer(L))
+        Hrows = (Wr[None] @ way_out(Y)).astype(np.complex64) if L >= 2 else Wr[None]
+        T = (Hrows @ G).astype(np.complex64)                  # T[e, out, in]
+        P2 = (np.abs(T) ** 2).astype(F32)
         P = np.zeros((n_e, 3, 3), F32)                          # P[e, in, out]
         P[:, :2, :2] = np.transpose(P2, (0, 2, 1))
         P[:, 0, 2] = np.clip(1 - P[:, 0, 0] - P[:, 0, 1], 0, 1)
@@ -104,6 +140,26 @@ def emulate(g, phase=phase_factor_m1):
         P[:, 2, 2] = np.clip(1 - P[:, 2, 0] - P[:, 2, 1], 0, 1)
         out[ic] = P.reshape(n_e, 9)
     return out
+
+
+def test_real_basis_is_real_and_fixes_the_electron_row():
+    """W is unitary, keeps e_0, and W^dagger M W is real for a CP-violating U; the same holds for
+    a U with arbitrary row and column phases (no phase convention assumed)."""
+    g = load_golden("prem12_io_dcp")
+    p = Propagator(1, 1)
+    p.setMNSMatrix(*g["params"][:4])
+    p.setNeutrinoMasses(*g["params"][4:])
+    U18 = np.asarray(p.Mix_U)
+    U = (U18[0::2] + 1j * U18[1::2]).reshape(3, 3)
+    rng = np.random.default_rng(1)
+    for Ux in (U, np.diag(np.exp(1j * rng.uniform(0, 6, 3))) @ U @ np.diag(np.exp(1j * rng.uniform(0, 6, 3)))):
+        Hr, W = real_basis(Ux, np.asarray(p.dm))
+        assert np.abs(np.conj(W.T) @ W - np.eye(3)).max() < 1e-15
+        assert np.abs(W[0] - [1, 0, 0]).max() == 0 and np.abs(W[:, 0] - [1, 0, 0]).max() == 0
+        M = sum(np.asarray(p.dm)[3 * n] * np.outer(Ux[:, n], np.conj(Ux[:, n])) for n in (1, 2))
+        scale = np.abs(M).max()
+        assert np.abs(Hr.imag).max() < 1e-14 * scale
+        assert np.abs(W @ Hr @ np.conj(W.T) - M).max() < 1e-14 * scale
 
 
 @pytest.mark.parametrize("name", golden_names())
@@ -123,4 +179,12 @@ def test_emulation_detects_a_wrong_quadrant():
     good = emulate(g)
     bad = emulate(g, phase=lambda t: phase_factor_m1(t + 1.0))
     assert np.abs(good - g["P"]).max() <= MIXED_TOL
+    assert np.abs(bad - g["P"]).max() > 1e-2
+
+
+def test_emulation_detects_a_wrong_way_out():
+    """The way out is the transpose of the way in (S_1 ... S_{L-1}); taking the product of the way
+    in itself (S_{L-1} ... S_1, the shells in the wrong order) breaks the bar."""
+    g = load_golden("prem59_nu")
+    bad = emulate(g, way_out=lambda Y: Y)
     assert np.abs(bad - g["P"]).max() > 1e-2
