@@ -379,7 +379,7 @@ __global__ __launch_bounds__(kBlock) void shell_spectrum_f64(GridDesc g, OscPara
 // lambda_b)), here of a real 3x3 matrix; v_k is the column of Q_k through its largest diagonal
 // entry, normalised (Q_k = v_k v_k^T).  Stored: v_1, v_2 in float as two float4 groups
 // Qm[((s*2 + g)*ne_pad + e)*4 + c] (packed index g*4 + c: 0-2 v_1, 3-5 v_2, 6-7 zero), and
-// dphi_k / L in quarter turns per km as Rm[(s*ne_pad + e)*2 + k - 1], double.
+// dphi_k / L in half turns per km as Rm[(s*ne_pad + e)*2 + k - 1], double.
 __device__ inline void real_eigenvectors_12(const OscParams& p, double E, double rho, double v[2][3],
                                             double mm[3][3], double mv[3][3]) {
     matter_masses(p, E, rho, mm, mv);
@@ -429,9 +429,9 @@ __device__ inline void spectrum_cell_mixed(const GridDesc& g, const OscParams& p
     dst[(std::size_t(s) * kMixedQGroups + 0) * st + e] =
         make_float4(float(v[0][0]), float(v[0][1]), float(v[0][2]), float(v[1][0]));
     dst[(std::size_t(s) * kMixedQGroups + 1) * st + e] = make_float4(float(v[1][1]), float(v[1][2]), 0.0f, 0.0f);
-    const double quarter_turn = 2.0 / (E * kPi);      // radians -> quarter turns
-    Rm[(std::size_t(s) * st + e) * 2 + 0] = -kLoE * mm[1][0] * quarter_turn;
-    Rm[(std::size_t(s) * st + e) * 2 + 1] = -kLoE * mm[2][0] * quarter_turn;
+    const double half_turn = 1.0 / (E * kPi);         // radians -> half turns
+    Rm[(std::size_t(s) * st + e) * 2 + 0] = -kLoE * mm[1][0] * half_turn;
+    Rm[(std::size_t(s) * st + e) * 2 + 1] = -kLoE * mm[2][0] * half_turn;
 }
 
 // Set-up of one mixed-precision grid in one launch: cells [0, G) are layer lengths

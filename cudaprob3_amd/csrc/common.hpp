@@ -71,7 +71,7 @@ hipError_t launch_geometry(const GridDesc& g, double* geom, hipStream_t s);
 //         -2.534 lambda_k (phase R*L/E).
 //  mixed: Qm float4 groups [(s*2 + g)*ne_pad + e] holding the real unit eigenvectors v_1, v_2
 //         of the layer Hamiltonian in the real basis (v_1, then v_2, then 2 zero),
-//         Rm[(s*ne_pad + e)*2 + k-1] = -2.534 (lambda_k - lambda_0) / E in quarter turns per km.
+//         Rm[(s*ne_pad + e)*2 + k-1] = -2.534 (lambda_k - lambda_0) / E in half turns per km.
 constexpr int kMixedQGroups = 2;
 constexpr int kMixedQFloats = 4 * kMixedQGroups;
 constexpr int kMixedRDoubles = 2;

@@ -4,7 +4,7 @@
 // propagate_pairs.hip.
 //   L    : shells crossed (maxlayers of the rows)
 //   gl[k]: layer lengths of row k in km, gl[k][0..L]
-//   Q, R : the spectrum table (float4 groups / double2 phase rates in quarter turns per km),
+//   Q, R : the spectrum table (float4 groups / double2 phase rates in half turns per km),
 //          row stride st
 //   ie   : energy index (< ne_pad)
 // Writes p[k][in][out] = P(in -> out) of row k.
@@ -33,33 +33,35 @@ __device__ inline Cx<float> cmul(Cx<float> a, Cx<float> b) {
     return r;
 }
 
-// e^{i phi} - 1 for a phase given in quarter turns (phi = t * pi/2), as (cos phi - 1, sin phi).
+// e^{i phi} - 1 for a phase given in half turns (phi = t * pi), as (cos phi - 1, sin phi).
 // Cody-Waite style reduction in FP64: q = rint(t), r = t - q in [-1/2, 1/2] is exact, so the
-// phase reaches FP32 only as the small remainder z = r * pi/2 in [-pi/4, pi/4]; there
-// sin z and cos z - 1 are Taylor polynomials (next terms below 3e-9 relative) evaluated in
-// FP32 by Horner's rule, and the quadrant q mod 4 rotates the result.
+// phase reaches FP32 only as the remainder z = r * pi in [-pi/2, pi/2]; there sin z and
+// cos z - 1 are polynomials evaluated in FP32 by Horner's rule, and e^{i q pi} = (-1)^q flips
+// the sign.  The coefficients are least-squares fits to sin and cos on [-pi/2, pi/2] with
+// Lawson's iterative reweighting towards the minimax error: |sin z - s0| <= 4.6e-9 and
+// |cos z - 1 - c0m1| <= 2.4e-10 before FP32 rounding (Taylor polynomials of the same degrees
+// would leave 3.6e-6).
 __device__ inline void phase_factor_m1(double t, float& cm1, float& sn) {
     // every fused multiply-add is written out and nothing else may be fused, so that each
     // inlined copy (one row, two rows, batched) rounds identically
 #pragma clang fp contract(off)
     const double q = rint(t);
-    const float z = float((t - q) * 1.5707963267948966);
+    const float z = float((t - q) * 3.141592653589793);
     const float z2 = z * z;
-    float ps = __builtin_fmaf(z2, 2.75573192e-6f, -1.98412698e-4f);
-    ps = __builtin_fmaf(z2, ps, 8.33333333e-3f);
-    ps = __builtin_fmaf(z2, ps, -1.66666667e-1f);
+    float ps = __builtin_fmaf(z2, 2.600053676e-6f, -1.980661473e-4f);
+    ps = __builtin_fmaf(z2, ps, 8.333017118e-3f);
+    ps = __builtin_fmaf(z2, ps, -1.666665673e-1f);
     const float s0 = __builtin_fmaf(z * z2, ps, z);
-    float pc = __builtin_fmaf(z2, -2.75573192e-7f, 2.48015873e-5f);
-    pc = __builtin_fmaf(z2, pc, -1.38888889e-3f);
-    pc = __builtin_fmaf(z2, pc, 4.16666667e-2f);
+    float pc = __builtin_fmaf(z2, -2.607709746e-7f, 2.476188638e-5f);
+    pc = __builtin_fmaf(z2, pc, -1.388840377e-3f);
+    pc = __builtin_fmaf(z2, pc, 4.166664183e-2f);
     pc = __builtin_fmaf(z2, pc, -0.5f);
     const float c0m1 = z2 * pc;
-    switch (int(q) & 3) {
-        case 0: cm1 = c0m1; sn = s0; break;                  // cos z, sin z
-        case 1: cm1 = -s0 - 1.0f; sn = c0m1 + 1.0f; break;   // -sin z, cos z
-        case 2: cm1 = -c0m1 - 2.0f; sn = -s0; break;         // -cos z, -sin z
-        default: cm1 = s0 - 1.0f; sn = -(c0m1 + 1.0f); break; // sin z, -cos z
-    }
+    // sign = (-1)^q from the low bit of q: e^{i phi} - 1 = sign (1 + c0m1 + i s0) - 1, i.e.
+    // (c0m1, s0) for even q and (-c0m1 - 2, -s0) for odd q, each with one rounding
+    const float sign = __builtin_bit_cast(float, (unsigned(int(q)) << 31) | 0x3f800000u);
+    cm1 = __builtin_fmaf(sign, c0m1, sign - 1.0f);
+    sn = sign * s0;
 }
 
 // One shell's table entries at this lane's energy: the real eigenvectors v_1, v_2 and the two
@@ -70,14 +72,17 @@ struct LayerTable {
     double2 r;
 };
 
-__device__ inline __attribute__((always_inline)) void load_table(const float4* __restrict__ qp,
-                                                                const double2* __restrict__ rp, std::size_t st,
-                                                                int i, LayerTable& t) {
-    const float4 a = qp[(std::size_t(i) * kMixedQGroups + 0) * st];
-    const float4 b = qp[(std::size_t(i) * kMixedQGroups + 1) * st];
+__device__ inline __attribute__((always_inline)) void load_table(const float4* __restrict__ Q,
+                                                                const double2* __restrict__ R, std::size_t st,
+                                                                unsigned off, int i, LayerTable& t) {
+    // wave-uniform row addresses plus one 32-bit lane offset (both element types are 16 bytes),
+    // so that the loads take the scalar-base form without 64-bit vector address arithmetic
+    const char* q0 = reinterpret_cast<const char*>(Q + std::size_t(i) * kMixedQGroups * st);
+    const float4 a = *reinterpret_cast<const float4*>(q0 + off);
+    const float4 b = *reinterpret_cast<const float4*>(q0 + st * sizeof(float4) + off);
     t.v[0][0] = a.x; t.v[0][1] = a.y; t.v[0][2] = a.z;
     t.v[1][0] = a.w; t.v[1][1] = b.x; t.v[1][2] = b.y;
-    t.r = rp[std::size_t(i) * st];
+    t.r = *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(R + std::size_t(i) * st) + off);
 }
 
 // w_k = e^{i dphi_k} - 1 of a layer of length dist km
@@ -159,17 +164,17 @@ __device__ inline void propagate_rows_mixed(int L, const double* const* gl, cons
                                             const double2* __restrict__ R, std::size_t st, int ie, float4 wmu,
                                             float (*p)[3][3]) {
 #pragma clang fp contract(off)
-    const float4* qp = Q + ie;
-    const double2* rp = R + ie;
+    static_assert(sizeof(float4) == 16 && sizeof(double2) == 16, "table entries are 16 bytes");
+    const unsigned off = unsigned(ie) * 16u;
     LayerTable t;
     Cx<float> Y[kRows][3][3];   // S_{i} ... S_1
     Cx<float> G[kRows][3][2];   // columns e, mu of ... S_0 W^dagger
     if (L >= 2) {
-        load_table(qp, rp, st, 1, t);
+        load_table(Q, R, st, off, 1, t);
 #pragma unroll
         for (int k = 0; k < kRows; k++) init_layer(t, layer_phases(t, gl[k][1]), Y[k]);
         auto shell = [&](int i) {
-            load_table(qp, rp, st, i, t);
+            load_table(Q, R, st, off, i, t);
 #pragma unroll
             for (int k = 0; k < kRows; k++) apply_layer<3>(t, layer_phases(t, gl[k][i]), Y[k]);
         };
@@ -185,7 +190,7 @@ __device__ inline void propagate_rows_mixed(int L, const double* const* gl, cons
     }
     // atmosphere: G = S_0 W^dagger[:, (e, mu)] = W^dagger[:, (e, mu)] + sum_k v_k w_k z_k^T,
     // z_k = W[(e, mu), :] v_k = (v_k0, W_mu1 v_k1 + W_mu2 v_k2) conjugated
-    load_table(qp, rp, st, 0, t);
+    load_table(Q, R, st, off, 0, t);
 #pragma unroll
     for (int k = 0; k < kRows; k++) {
         const LayerPhases w = layer_phases(t, gl[k][0]);
@@ -230,7 +235,7 @@ __device__ inline void propagate_rows_mixed(int L, const double* const* gl, cons
         }
     }
     if (L >= 1) {
-        load_table(qp, rp, st, L, t);
+        load_table(Q, R, st, off, L, t);
 #pragma unroll
         for (int k = 0; k < kRows; k++) apply_layer<2>(t, layer_phases(t, gl[k][L]), G[k]);
     }
