@@ -409,12 +409,15 @@ __device__ inline void real_eigenvectors_12(const OscParams& p, double E, double
                 }
                 Q[n][m] = Q[m][n] = acc * inv;
             }
-        int j = 0;
-        if (Q[1][1] > Q[j][j]) j = 1;
-        if (Q[2][2] > Q[j][j]) j = 2;
-        const double s = Q[j][j] > 0.0 ? 1.0 / sqrt(Q[j][j]) : 0.0;
+        // the column through the largest diagonal entry, picked by selects: an index into Q that
+        // is not a compile-time constant makes the compiler keep Q in LDS
+        const bool j1 = Q[1][1] > Q[0][0];
+        const double d01 = j1 ? Q[1][1] : Q[0][0];
+        const bool j2 = Q[2][2] > d01;
+        const double djj = j2 ? Q[2][2] : d01;
+        const double s = djj > 0.0 ? 1.0 / sqrt(djj) : 0.0;
 #pragma unroll
-        for (int n = 0; n < 3; n++) v[kk][n] = Q[n][j] * s;
+        for (int n = 0; n < 3; n++) v[kk][n] = (j2 ? Q[n][2] : (j1 ? Q[n][1] : Q[n][0])) * s;
     }
 }
 
