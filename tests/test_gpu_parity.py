@@ -175,6 +175,31 @@ def test_independent_handles_do_not_share_parameters(oracle_mod):
         p.close()
 
 
+@pytest.mark.parametrize("name", ["prem12_io_dcp", "prem12_io_dcp_nubar", "prem59_nu"])
+def test_mixed_any_phase_convention(name):
+    """The mixed kernels' real basis is derived from U itself (host_util.hpp real_basis), so a U
+    with arbitrary row and column phases -- the same physics -- gives the same probabilities:
+    within the mixed bar of the golden P for the PDG U."""
+    g = load_golden(name)
+    rng = np.random.default_rng(11)
+    cos, E = g["cosines"], g["energies"]
+    with HipPropagatorSingle(0, len(cos), len(E), precision="mixed") as p:
+        p.setEnergyList(E); p.setCosineList(cos)
+        p.setDensity(g["radii"], g["rhos"]); p.setProductionHeight(float(g["h_km"]))
+        p.setMNSMatrix(*g["params"][:4]); p.setNeutrinoMasses(*g["params"][4:])
+        U18 = np.asarray(p.Mix_U, dtype=np.float64)
+        U = (U18[0::2] + 1j * U18[1::2]).reshape(3, 3)
+        V = np.diag(np.exp(1j * rng.uniform(0, 6.3, 3))) @ U @ np.diag(np.exp(1j * rng.uniform(0, 6.3, 3)))
+        assert np.abs(V - U).max() > 0.1
+        p.Mix_U = list(np.stack([V.real, V.imag], -1).reshape(18))
+        p._params_uploaded = False
+        print(f"[gpu] rephased U, {name}", flush=True)
+        p.calculateProbabilities(Antineutrino if int(g["antinu"]) else Neutrino)
+        err = np.abs(p.getProbabilities().transpose(1, 2, 0) - g["P"]).max()
+    print(f"[gpu]   max|dP| {err:.3e}")
+    assert err <= 1e-5
+
+
 def test_float32_api_runs_mixed(oracle_mod):
     g = W.prem4_256()
     cos, E = g.cosines[::8].astype(np.float32), g.energies[::8].astype(np.float32)
