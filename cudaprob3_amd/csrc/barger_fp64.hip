@@ -7,11 +7,14 @@
 //                            cosine row x 64 energies, the AXFAC table in LDS, parameters
 //                            by value in the kernel-argument segment.
 //  * layer_geometry       -- per-cosine layer lengths (physics.hpp:553-580, 615-618).
-//  * shell_spectrum_*     -- per-(shell, energy) eigenvalues and flavour-basis projectors.
+//  * shell_spectrum_f64   -- per-(shell, energy) eigenvalues and flavour-basis projectors.
 //                            A layer's amplitude is U (sum_k e^{i phi_k} P_k) U^dagger with
 //                            P_k the reference's eq. 11 products (physics.hpp:346-431); the
 //                            P_k depend on (E, rho) only, so they are computed once per
 //                            (shell, energy) instead of once per (pair, layer).
+//  * spectrum_cell_mixed  -- the same spectrum for the mixed kernels, as real eigenvectors in
+//                            the real basis of OscParams::Hr (set-up kernels grid_setup_mixed
+//                            and batch_setup).
 #include "common.hpp"
 
 namespace cp3 {
