@@ -200,6 +200,29 @@ def test_mixed_any_phase_convention(name):
     assert err <= 1e-5
 
 
+@pytest.mark.parametrize("params", [(0.5836, 0.0, 0.7222, 3.86, 7.42e-5, 2.494e-3),
+                                    (0.0, 0.0, 0.7222, 3.86, 7.42e-5, -2.494e-3),
+                                    (0.5836, 0.1496, 0.0, 0.0, 7.42e-5, 2.494e-3)])
+def test_mixed_real_basis_edge_angles(oracle_mod, params):
+    """Mixing angles at which the real basis has a free phase (theta13 = 0: (W^dagger M W)_12 = 0)
+    or no first-column direction (theta12 = theta13 = 0: M_10 = M_20 = 0), and theta23 = 0:
+    the mixed kernel against the oracle."""
+    g = W.prem4_256()
+    cos, E = g.cosines[::8], g.energies[::8]
+    print(f"[gpu] edge angles {params}", flush=True)
+    for antinu in (0, 1):
+        with HipPropagatorSingle(0, len(cos), len(E), precision="mixed") as p:
+            p.setEnergyList(E); p.setCosineList(cos)
+            p.setMNSMatrix(*params[:4]); p.setNeutrinoMasses(*params[4:])
+            p.setDensity(g.radii, g.rhos); p.setProductionHeight(22.0)
+            p.calculateProbabilities(Antineutrino if antinu else Neutrino)
+            P = p.getProbabilities().transpose(1, 2, 0)
+        ref = oracle_mod.propagate(antinu, cos, E, g.radii, g.rhos, 22.0, params)
+        err = np.abs(P - ref).max()
+        print(f"[gpu]   antinu {antinu}: max|dP| {err:.3e}")
+        assert err <= 1e-5
+
+
 def test_float32_api_runs_mixed(oracle_mod):
     g = W.prem4_256()
     cos, E = g.cosines[::8].astype(np.float32), g.energies[::8].astype(np.float32)
