@@ -124,6 +124,27 @@ def test_jitter12_2048_full_size(oracle_mod, precision):
         assert uni <= (1e-11 if precision == "fp64" else 10 * MIXED_TOL)
 
 
+@pytest.mark.parametrize("antinu,inverted", [(0, False), (1, False), (0, True)])
+def test_fine100_mixed_long_paths(oracle_mod, antinu, inverted):
+    """The 100-shell Earth of the Markov-chain workload, whose core-crossing paths have up to 199
+    layers (the longest running products of any workload): the mixed kernel on a 512 x 512 grid
+    over the workload's ranges, all rows against the FP64 kernel and strided rows against the
+    oracle, within the mixed bar."""
+    g = W.fine100_grid(512)
+    params = list(g.params)
+    if inverted:
+        params[5] = -params[5]
+    print(f"[gpu] fine100 512^2 mixed antinu={antinu} dm32={params[5]}", flush=True)
+    P = run_gpu("mixed", g.cosines, g.energies, g.radii, g.rhos, params, g.h_km, antinu)
+    P64 = run_gpu("fp64", g.cosines, g.energies, g.radii, g.rhos, params, g.h_km, antinu)
+    rows = np.arange(0, 512, 29)
+    ref = oracle_mod.propagate(antinu, g.cosines[rows], g.energies, g.radii, g.rhos, g.h_km, params)
+    e64 = np.abs(P - P64)
+    eref = np.abs(P[rows] - ref).max()
+    print(f"[gpu]   vs fp64 kernel max {e64.max():.3e} rms {np.sqrt((e64 ** 2).mean()):.3e}; vs oracle {eref:.3e}")
+    assert e64.max() <= MIXED_TOL and eref <= MIXED_TOL
+
+
 def test_api_errors_and_state():
     with pytest.raises(RuntimeError, match="not available"):
         HipPropagatorSingle(999, 4, 4)
