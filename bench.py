@@ -44,7 +44,7 @@ def make_prop(g, precision):
 
 
 def time_steps(props, grids, steps, warmup, sync):
-    """One step = every variant launched (in order, each on its propagator's stream), then waited for."""
+    """One step = every variant launched (in order, on the propagators' stream), then waited for."""
     def step():
         for p, g in zip(props, grids):
             p.calculateProbabilitiesAsync(g.antinu)
@@ -531,10 +531,14 @@ def main():
     pairs_per_step = sum(g.n_pairs for g in grids)
     log(f"[bench] workload {workload}: {len(grids)} variants x {grids[0].n_pairs} pairs, precision {args.precision}")
 
-    # each propagator on its own stream (default priority), as each CudaPropagatorSingle has its
-    # own (cudapropagator.cuh:70): one variant's set-up kernel can then run beside the previous
-    # variant's pair kernel (0.24 against 0.255 ms per step on one stream)
+    # the four propagators share one stream (default priority), so their kernels run one after
+    # the other and every kernel duration is its own: with one stream per propagator the steps
+    # are 5 % shorter, but the pair kernels of different variants overlap and their durations
+    # (HIP events, rocprof) then include each other's work
     props = [make_prop(g, args.precision) for g in grids]
+    shared = torch.cuda.Stream()
+    for p in props:
+        p.set_stream(shared.cuda_stream)
     dt = time_steps(props, grids, args.steps, args.warmup, sync)
     value = args.steps * pairs_per_step / dt
     log(f"[bench] {args.steps} steps in {dt:.4f} s -> {value:.4e} pairs/s")
@@ -604,6 +608,7 @@ def main():
         log(f"[bench] CPU baseline {result['cpu_baseline']['value']:.4e} pairs/s")
     for p in props:
         p.close()
+    del shared
     print(json.dumps(result), flush=True)
 
 
