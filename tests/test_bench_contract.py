@@ -1,0 +1,46 @@
+"""bench.py's contract pieces that run without a GPU: the single-GPU refusal, the command line,
+and the algorithmic work accounting its roofline divides by."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+from cudaprob3_amd import metrics as M
+from cudaprob3_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_is_refused_with_a_message():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2", "--steps", "1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "single MI355X" in r.stderr and r.stdout == ""
+
+
+def test_help_lists_the_workloads():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0
+    for w in ("jitter12", "scan1024", "mcmc8192", "surrogate2048"):
+        assert w in r.stdout
+
+
+def test_flops_per_pair_from_its_parts():
+    """F(L) from the reference's structure (physics.hpp:645-699): L+1 layer steps of 1731 flops,
+    2L-1 running-product multiplies in the loop plus the final one (L >= 1) or none plus the
+    final one (L = 0), 216 flops each, and 27 flops for the nine |A|^2."""
+    L = np.arange(0, 120)
+    mults = np.where(L == 0, 1, 2 * L)
+    expect = M.FLOPS_LAYER_STEP * (L + 1) + M.FLOPS_MATMUL * mults + 27
+    assert np.array_equal(M.flops_per_pair(L), expect)
+
+
+def test_jitter12_grid_flops():
+    """The headline grid's algorithmic work per launch (the roofline's numerator): 34.60 GFLOP,
+    from the host-computed maxlayers of the jittered 12-shell Earth."""
+    g = W.jitter12_2048()
+    ml = M.maxlayers_of(g.cosines, g.radii, g.rhos)
+    assert ml.max() == 12 and (ml == 0).sum() == 1024
+    assert abs(M.grid_flops(ml, len(g.energies)) / 1e9 - 34.60) < 0.005
