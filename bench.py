@@ -109,12 +109,12 @@ def cpu_baseline(grids, threads):
                       f"pairs, CpuPropagator<double> algorithm, OpenMP dynamic over cosines, {dt:.1f} s"}
 
 
-def traffic_from_profile(workload, precision):
+def traffic_from_profile(workload, precision, key="hbm_bytes_per_launch"):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{precision}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        return json.load(f).get(key)
 
 
 def run_scan(args, sync):
@@ -575,6 +575,14 @@ def main():
                      "algorithmic_flops_per_launch": flops,
                      "output_GBps": out_bytes / (k_ms * 1e-3) / 1e9},
     }
+    # what the hardware executed (rocprof instruction counters of the same kernel, committed under
+    # profiles/), over the live kernel time: the utilisation figure, where `frac` above counts the
+    # reference algorithm's work
+    executed = traffic_from_profile(workload, args.precision, "executed_gflop")
+    if executed:
+        ex_tflops = executed * 1e9 / (k_ms * 1e-3) / 1e12
+        result["roofline"].update({"executed_flops_per_launch": executed * 1e9, "executed_TFLOPs": ex_tflops,
+                                   "executed_frac_of_nonpacked_fp32_peak": ex_tflops / M.PEAK_FP32_NONPACKED_TFLOPS})
 
     if not args.no_accuracy and args.precision == "mixed":
         worst, rms = accuracy(grids, props)

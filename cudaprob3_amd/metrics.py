@@ -21,6 +21,7 @@ FLOPS_MATMUL = 216
 
 # MI355X published peaks
 PEAK_FP32_VALU_TFLOPS = 157.3      # vector FP32 (with packed FP32)
+PEAK_FP32_NONPACKED_TFLOPS = 78.6  # vector FP32 without packed instructions (disabled in this build)
 PEAK_FP64_VALU_TFLOPS = 78.6       # vector FP64
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_MFMA_TFLOPS = 2500.0
