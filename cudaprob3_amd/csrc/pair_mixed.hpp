@@ -57,9 +57,14 @@ __device__ inline void phase_factor_m1(double t, float& cm1, float& sn) {
     pc = __builtin_fmaf(z2, pc, 4.166664183e-2f);
     pc = __builtin_fmaf(z2, pc, -0.5f);
     const float c0m1 = z2 * pc;
-    // sign = (-1)^q from the low bit of q: e^{i phi} - 1 = sign (1 + c0m1 + i s0) - 1, i.e.
-    // (c0m1, s0) for even q and (-c0m1 - 2, -s0) for odd q, each with one rounding
-    const float sign = __builtin_bit_cast(float, (unsigned(int(q)) << 31) | 0x3f800000u);
+    // sign = (-1)^q: e^{i phi} - 1 = sign (1 + c0m1 + i s0) - 1, i.e. (c0m1, s0) for even q and
+    // (-c0m1 - 2, -s0) for odd q, each with one rounding.  The parity of q is the last mantissa
+    // bit of q + 1.5 * 2^52: for |q| < 2^51 the sum lies in [2^52, 2^53), where the last bit is
+    // worth 1, and the added even integer keeps the parity (2^52 alone would not do for q < 0:
+    // the sum then falls below 2^52, where the last bit is worth 1/2).  No 32-bit integer
+    // conversion, so phases up to 2^51 half turns are handled.
+    const unsigned lo = unsigned(__builtin_bit_cast(unsigned long long, q + 6755399441055744.0));
+    const float sign = __builtin_bit_cast(float, (lo << 31) | 0x3f800000u);
     cm1 = __builtin_fmaf(sign, c0m1, sign - 1.0f);
     sn = sign * s0;
 }

@@ -61,6 +61,22 @@ SIN_C = [F32(c) for c in (-1.666665673e-1, 8.333017118e-3, -1.980661473e-4, 2.60
 COS_C = [F32(c) for c in (-0.5, 4.166664183e-2, -1.388840377e-3, 2.476188638e-5, -2.607709746e-7)]
 
 
+def sign_bit(q):
+    """The kernels' parity of the integer-valued FP64 q: the last mantissa bit of q + 1.5 * 2^52."""
+    return (np.asarray(q, np.float64) + 6755399441055744.0).view(np.uint64) & 1
+
+
+def test_sign_bit_is_the_parity():
+    """The parity trick of phase_factor_m1 (pair_mixed.hpp) over negative and positive q up to
+    2^51, against integer parity; the 2^52 variant fails for negative odd q."""
+    rng = np.random.default_rng(3)
+    q = np.concatenate([np.arange(-5000, 5000), rng.integers(-2 ** 51 + 1, 2 ** 51, 200_000),
+                        [2 ** 51 - 1, -(2 ** 51) + 1, 2 ** 31, -(2 ** 31) - 1]]).astype(np.float64)
+    assert np.array_equal(sign_bit(q), q.astype(np.int64) & 1)
+    wrong = (q + 4503599627370496.0).view(np.uint64) & 1
+    assert not np.array_equal(wrong, q.astype(np.int64) & 1)
+
+
 def phase_factor_m1(t):
     """e^{i phi} - 1 for phases t in half turns: FP64 reduction, float32 polynomials, sign (-1)^q."""
     q = np.rint(t)
@@ -74,7 +90,7 @@ def phase_factor_m1(t):
     for c in COS_C[-2::-1]:
         pc = z2 * pc + c
     c0m1 = z2 * pc
-    sign = np.where(np.asarray(q, np.int64) & 1, F32(-1), F32(1)).astype(F32)
+    sign = np.where(sign_bit(q), F32(-1), F32(1)).astype(F32)
     cm1 = sign * c0m1 + (sign - F32(1))
     sn = sign * s0
     return (cm1.astype(F32) + 1j * sn.astype(F32)).astype(np.complex64)
