@@ -148,9 +148,10 @@ def test_fine100_mixed_long_paths(oracle_mod, antinu, inverted):
 def test_mixed_extreme_phases():
     """Energies down to 1e-8 GeV on Earth-crossing paths give phase differences of up to ~10^10 rad
     (more than 2^31 half turns): the mixed kernel's sign (-1)^q must still follow the parity of
-    q.  Against the FP64 kernel (same FP64 phase range) within the mixed bar."""
+    q.  Energies up to 1e6 GeV give tiny phases and matter-dominated spectra.  Against the FP64
+    kernel (same FP64 phase range) within the mixed bar."""
     cos = np.array([-1.0, -0.83, -0.5, -0.1, 0.3])
-    E = np.array([1e-8, 2.3e-8, 1e-7, 1e-6, 1e-4, 1e-2, 1.0])
+    E = np.array([1e-8, 2.3e-8, 1e-7, 1e-6, 1e-4, 1e-2, 1.0, 1e3, 1e4, 1e5, 1e6])
     print("[gpu] mixed, extreme phases", flush=True)
     for antinu in (0, 1):
         P = run_gpu("mixed", cos, E, W.PREM12[:, 0], W.PREM12[:, 1], W.NUFIT, 22.0, antinu)
