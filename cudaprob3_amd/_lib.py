@@ -53,6 +53,10 @@ SIGNATURES = {
     "cp3h_last_compute_ms": (ctypes.c_int, [_handle, _c_float_p, _c_float_p]),
     "cp3h_set_timing": (ctypes.c_int, [_handle, ctypes.c_int]),
     "cp3h_last_error": (ctypes.c_char_p, []),
+    "cp3h_propagate_mixed_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                    ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.c_int, _c_double_p, _c_double_p, ctypes.c_int,
+                                                    ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
     "cp3h_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p,
                                          ctypes.POINTER(_handle)]),
     "cp3h_batch_destroy": (ctypes.c_int, [_handle]),

@@ -284,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void propagate_plain_f64(GridDesc g, OscPar
 
 // Length in km of layer i of cosine row ic (0 beyond the row's crossed layers).
 __device__ inline double geometry_cell(const GridDesc& g, int ic, int i) {
-    const int L = g.maxlayers[ic];
+    const int L = crossed_shells(g.maxlayers, ic, g.rows);
     if (i > L) return 0.0;
     const double cz = g.cosines[ic];
     const double path = path_length(cz, g.h_cm);

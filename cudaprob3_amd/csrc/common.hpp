@@ -43,6 +43,14 @@ __device__ inline unsigned xcd_logical_block(unsigned b, unsigned nblocks) {
 
 inline unsigned round_up(unsigned x, unsigned m) { return (x + m - 1) / m * m; }
 
+// Shells crossed by cosine row ic, kept in [0, rows] so that no table, layer-length or radius
+// read leaves its array whatever the caller passed (entry points that validate maxlayers on the
+// host never trigger the clamp; cp3h_propagate_mixed_noalloc takes maxlayers in device memory
+// and cannot validate them).
+__device__ inline int crossed_shells(const int* maxlayers, int ic, int rows) {
+    return min(max(maxlayers[ic], 0), rows);
+}
+
 // ---- launchers (defined in the .hip translation units) --------------------------------
 
 struct GridDesc {

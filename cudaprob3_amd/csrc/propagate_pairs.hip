@@ -230,8 +230,8 @@ __global__ __launch_bounds__(kBlock) void pairs_mixed_kernel(MixedArgs a) {
                 for (int ou = 0; ou < 3; ou++) __builtin_nontemporal_store(p[in][ou], o + (in * 3 + ou) * plane);
         }
     };
-    const int L0 = n_rows > 0 ? a.maxlayers[ic0] : 0;
-    if (n_rows == 2 && a.maxlayers[ic0 + 1] == L0) {
+    const int L0 = n_rows > 0 ? crossed_shells(a.maxlayers, ic0, a.rows) : 0;
+    if (n_rows == 2 && crossed_shells(a.maxlayers, ic0 + 1, a.rows) == L0) {
         const double* gl[2] = {a.geom + std::size_t(ic0) * (a.rows + 1), a.geom + std::size_t(ic0 + 1) * (a.rows + 1)};
         float p[2][3][3];
         propagate_rows_mixed<2>(L0, gl, a.Q, a.R, st, ie, a.wmu, p);
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void pairs_mixed_kernel(MixedArgs a) {
             const int ic = ic0 + k;
             const double* gl[1] = {a.geom + std::size_t(ic) * (a.rows + 1)};
             float p[1][3][3];
-            propagate_rows_mixed<1>(a.maxlayers[ic], gl, a.Q, a.R, st, ie, a.wmu, p);
+            propagate_rows_mixed<1>(crossed_shells(a.maxlayers, ic, a.rows), gl, a.Q, a.R, st, ie, a.wmu, p);
             emit(ic, p[0]);
         }
     }

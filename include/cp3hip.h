@@ -134,6 +134,29 @@ int cp3h_set_timing(cp3h_propagator* h, int enable);
 /* Message of the last failing call on this thread ("" if none). */
 const char* cp3h_last_error(void);
 
+/* Allocation-free propagation.  No counterpart in the reference: for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc).  It is cp3h_compute_async of a CP3H_MIXED handle without the handle: the
+   grid, the Earth model and the results are caller device memory, the parameters are host
+   arrays as for cp3h_set_params_f64, and the kernels' scratch (layer lengths and the spectrum
+   table) lives in one static device array of 64 MiB per GPU owned by the library.
+     cosines_device[n_cosines], maxlayers_device[n_cosines] (each in [0, n_rows]; entries outside
+     are clamped into it, so no read leaves the arrays, but such rows' results are meaningless),
+     energies_device[n_energies] (GeV), radii_desc_device / rhos_desc_device[n_rows] (descending,
+     as for cp3h_set_density_f64), out_device: float SoA [9][n_cosines][n_energies].
+   Enqueues two kernels on `stream` (a hipStream_t as void*, NULL = the null stream) and returns;
+   calls no hipMalloc, hipFree or other allocating routine.  Limit: 8 n_cosines (n_rows + 1) +
+   48 (n_rows + 1) n_energies_padded bytes <= 64 MiB (n_energies_padded = n_energies rounded up
+   to 64); larger grids are refused with CP3H_ERR_ARGUMENT.  All calls on one GPU share its
+   scratch: calls whose device work overlaps (different streams, no ordering) race; calls on one
+   stream are ordered.  The first call on a device looks the scratch's address up (make it
+   outside graph capture). */
+int cp3h_propagate_mixed_noalloc(int device, const double* cosines_device, const int* maxlayers_device,
+                                 int n_cosines, const double* energies_device, int n_energies,
+                                 const double* radii_desc_device, const double* rhos_desc_device, int n_rows,
+                                 const double* U18, const double* DM9, int antineutrino,
+                                 double production_height_cm, float* out_device, void* stream);
+
 /* ---- Batched parameter scans ------------------------------------------------------------
  * No counterpart in the reference, whose propagators hold one hypothesis, one grid and one
  * Earth model each: here many parameter sets, each with its own grid and Earth model, are
