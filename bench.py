@@ -1,20 +1,32 @@
 #!/usr/bin/env python3
 """Throughput of the propagation path on one MI355X.
 
-Workload (default): 2048 cosZ in [-1, 1] x 2048 energies log-spaced 0.05-1000 GeV on the
-12-shell jittered PREM-like Earth; one step = the four variants nu/nubar x normal/inverted
-ordering, i.e. four calculateProbabilities of the full grid (16.8 M pairs), inputs already
-resident in HBM, results left in HBM (SoA, FP32 for the mixed kernel).
+Headline workload (default): 2048 cosZ in [-1, 1] x 2048 energies log-spaced 0.05-1000 GeV on
+the 12-shell jittered PREM-like Earth; one step = the four variants nu/nubar x normal/inverted
+ordering, i.e. four calculateProbabilities of the full grid (16.8 M pairs) through the drop-in
+API, each with the reference's per-call parameter hand-over (cudapropagator.cuh:191-220), inputs
+already resident in HBM, results left in HBM (SoA, FP32 for the mixed kernel).
 
-Prints one JSON line.  value = pairs/s of the whole step sequence; roofline = algorithmic
-flops of the reference algorithm per launch / average duration of the per-pair kernel
-(HIP events on its stream); cpu_baseline = the reference CpuPropagator<double> (or the
-oracle restatement if the reference build is absent) on the same four grids with all
-host threads of the box's share.
+Prints one JSON line.
+  value     pairs/s of the whole step sequence (host launch overhead included).
+  roofline  the pair kernel: flops of the algorithm it runs (cudaprob3_amd/metrics.py,
+            within ~1 % of rocprofv3's instruction counters) per launch / the average duration
+            of one launch, from back-to-back launches between HIP events on its stream
+            (cp3h_time_kernels); the reference algorithm's flop rate for the same results is
+            `reference_work_TFLOPs`.
+  scan1024, mcmc8192, surrogate
+            the batched parameter scan, the Markov chain over the 8192^2 grid on 100 shells and
+            the learned-surrogate comparison, each with its own roofline and CPU baseline.
+  cpu_baseline
+            the reference CpuPropagator<double> (or the oracle restatement if the reference
+            build is absent) on the same four grids, with the host threads of the box's share;
+            the same leg compares every GPU result with the reference's FP64 values
+            (accuracy_vs_reference_fp64).
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -30,6 +42,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_threads() -> int:
+    """Threads for the CPU baselines: every CPU this process may run on (its affinity set),
+    capped by OMP_NUM_THREADS when that is set (the CPU share the box grants one GPU)."""
+    aff = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return min(aff, int(env)) if env.isdigit() and int(env) > 0 else aff
+
+
+def cpu_fields(threads: int) -> dict:
+    return {"cores": threads, "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+def reference_fn():
+    import oracle
+    kind = "reference" if oracle.ref_available() else "port"
+    return kind, (oracle.ref_propagate if kind == "reference" else oracle.propagate)
+
+
 def make_prop(g, precision):
     """A propagator holding one variant's grid, Earth model and hypothesis."""
     from cudaprob3_amd.propagator import HipPropagatorSingle
@@ -43,8 +74,10 @@ def make_prop(g, precision):
     return p
 
 
-def time_steps(props, grids, steps, warmup, sync):
-    """One step = every variant launched (in order, on the propagators' stream), then waited for."""
+def time_steps(props, grids, steps, warmup, sync, after_warmup=None):
+    """One step = every variant's calculateProbabilitiesAsync (parameter hand-over + launches, in
+    order, on the propagators' stream), then waitForCompletion on each.  after_warmup() runs
+    between the warm-up and the timed steps (untimed)."""
     def step():
         for p, g in zip(props, grids):
             p.calculateProbabilitiesAsync(g.antinu)
@@ -52,6 +85,8 @@ def time_steps(props, grids, steps, warmup, sync):
             p.waitForCompletion()
     for _ in range(warmup):
         step()
+    if after_warmup is not None:
+        after_warmup()
     sync()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -60,53 +95,49 @@ def time_steps(props, grids, steps, warmup, sync):
     return time.perf_counter() - t0
 
 
-def kernel_ms(props, grids, reps):
-    """Average GPU duration of the per-pair kernel and of all kernels, per launch, with one
-    launch in flight at a time."""
-    pairs, total = [], []
-    for p in props:
-        p.set_timing(True)
+def kernel_times(props, grids, reps):
+    """Average duration of one set-up and one pair-kernel launch over the variants, each kernel
+    launched `reps` times back to back between two HIP events (cp3h_time_kernels), after one
+    untimed round (the clocks of an idle GPU take a few launches to come up)."""
+    for p, g in zip(props, grids):
+        p.time_kernels(g.antinu, 2)
+    setup, pairs = [], []
+    for p, g in zip(props, grids):
+        su, pa = p.time_kernels(g.antinu, reps)
+        setup.append(su)
+        pairs.append(pa)
+    return float(np.mean(setup)), float(np.mean(pairs))
+
+
+def d2h_times(prop, reps=5):
+    """Copy-back of one grid's results into the handle's pinned host buffer
+    (cp3h_fetch_results, the lazy copy of the first getProbability): best of `reps`."""
+    from cudaprob3_amd import _lib
+    lib = _lib.load()
+    ptr, nbytes = ctypes.c_void_p(), ctypes.c_int()
+    best = float("inf")
     for _ in range(reps):
-        for p, g in zip(props, grids):
-            p.calculateProbabilities(g.antinu)
-            t, k = p.last_compute_times()
-            total.append(t)
-            pairs.append(k)
-    for p in props:
-        p.set_timing(False)
-    return float(np.mean(pairs)), float(np.mean(total))
+        t0 = time.perf_counter()
+        _lib.check(lib.cp3h_fetch_results(prop._handle(), ctypes.byref(ptr), ctypes.byref(nbytes)))
+        best = min(best, time.perf_counter() - t0)
+    size = 9 * prop.n_cosines * prop.n_energies * nbytes.value
+    return best * 1e3, size
 
 
-def accuracy(grids, props):
-    """max / rms |dP| of the mixed kernel against the FP64 kernel (pinned to the reference
-    within 1e-12 by the tests) over the full grids."""
+def accuracy_vs_fp64_kernel(grids, props):
+    """max / rms |dP| of the mixed kernel against the FP64 kernel over the full grids (the FP64
+    kernel is itself held within 1e-12 of the reference by the parity tests)."""
     worst, sq, n = 0.0, 0.0, 0
     for g, p_mixed in zip(grids, props):
         p64 = make_prop(g, "fp64")
         p64.calculateProbabilities(g.antinu)
         p_mixed.calculateProbabilities(g.antinu)
         d = np.abs(p64.getProbabilities() - p_mixed.getProbabilities().astype(np.float64))
-        p_mixed.resultsResideOnHost = False
         p64.close()
         worst = max(worst, float(d.max()))
         sq += float((d * d).sum())
         n += d.size
     return worst, float(np.sqrt(sq / n))
-
-
-def cpu_baseline(grids, threads):
-    import oracle
-    kind = "reference" if oracle.ref_available() else "port"
-    fn = oracle.ref_propagate if kind == "reference" else oracle.propagate
-    t0 = time.perf_counter()
-    pairs = 0
-    for g in grids:
-        fn(g.antinu, g.cosines, g.energies, g.radii, g.rhos, g.h_km, g.params, threads=threads)
-        pairs += g.n_pairs
-    dt = time.perf_counter() - t0
-    return {"value": pairs / dt, "unit": "pairs/s", "cores": threads, "kind": kind,
-            "sample": f"one full step: {len(grids)} grids of {len(grids[0].cosines)}x{len(grids[0].energies)} "
-                      f"pairs, CpuPropagator<double> algorithm, OpenMP dynamic over cosines, {dt:.1f} s"}
 
 
 def traffic_from_profile(workload, precision, key="hbm_bytes_per_launch"):
@@ -117,7 +148,143 @@ def traffic_from_profile(workload, precision, key="hbm_bytes_per_launch"):
         return json.load(f).get(key)
 
 
-def run_scan(args, sync):
+def cpu_baseline_and_accuracy(grids, gpu_results, threads):
+    """The reference CpuPropagator<double> on the same four grids (timed), and every GPU result
+    of the mixed kernel (gpu_results, SoA per grid; None: timing only) against the reference's
+    FP64 values."""
+    kind, fn = reference_fn()
+    dt, worst, sq, n = 0.0, 0.0, 0.0, 0
+    for i, g in enumerate(grids):
+        t0 = time.perf_counter()
+        ref = fn(g.antinu, g.cosines, g.energies, g.radii, g.rhos, g.h_km, g.params, threads=threads)
+        dt += time.perf_counter() - t0
+        if gpu_results is None:
+            continue
+        d = np.abs(gpu_results[i].transpose(1, 2, 0).astype(np.float64) - ref)      # [ic, ie, t]
+        worst = max(worst, float(d.max()))
+        sq += float(np.einsum("ijk,ijk->", d, d))
+        n += d.size
+        del ref, d
+    pairs = sum(g.n_pairs for g in grids)
+    base = {"value": pairs / dt, "unit": "pairs/s", **cpu_fields(threads), "kind": kind,
+            "sample": f"one full step: {len(grids)} grids of {len(grids[0].cosines)}x{len(grids[0].energies)} "
+                      f"pairs, CpuPropagator<double> algorithm, OpenMP dynamic over cosines, {dt:.1f} s"}
+    if gpu_results is None:
+        return base, None
+    acc = {"max_abs_dP": worst, "rms_dP": float(np.sqrt(sq / n)), "pairs": pairs, "channels": 9,
+           "against": f"{kind} CpuPropagator<double>, every pair of the step"}
+    return base, acc
+
+
+def run_jitter12(args, sync):
+    """The headline line: the four 2048^2 variants per step."""
+    from cudaprob3_amd import metrics as M
+    from cudaprob3_amd import workloads as W
+    import torch
+
+    grids = W.jitter12_variants(args.n)
+    workload = f"jitter12_{args.n}"
+    pairs_per_step = sum(g.n_pairs for g in grids)
+    log(f"[bench] workload {workload}: {len(grids)} variants x {grids[0].n_pairs} pairs, precision {args.precision}")
+    # the four propagators share one stream (default priority), so their kernels run one after
+    # the other and every kernel duration is its own: with one stream per propagator the steps
+    # are 5 % shorter, but the pair kernels of different variants overlap and their durations
+    # (HIP events, rocprof) then include each other's work
+    props = [make_prop(g, args.precision) for g in grids]
+    shared = torch.cuda.Stream()
+    for p in props:
+        p.set_stream(shared.cuda_stream)
+
+    # kernel durations between the warm-up and the timed steps: back-to-back launches, which also
+    # keep the clocks up for the timed steps
+    times = {}
+    dt = time_steps(props, grids, args.steps, args.warmup, sync,
+                    after_warmup=lambda: times.update(k=kernel_times(props, grids, args.kernel_reps)))
+    setup_ms, k_ms = times["k"]
+    value = args.steps * pairs_per_step / dt
+    log(f"[bench] {args.steps} steps in {dt:.4f} s -> {value:.4e} pairs/s")
+
+    ml = M.maxlayers_of(grids[0].cosines, grids[0].radii, grids[0].rhos)
+    ne = len(grids[0].energies)
+    elem = 4 if args.precision == "mixed" else 8
+    out_bytes = elem * 9 * grids[0].n_pairs
+    ref_flops = M.grid_flops(ml, ne)
+    roof = {"bound": "valu"}
+    if args.precision == "mixed":
+        f32, f64 = M.mixed_grid_flops(ml, ne)
+        roof.update(M.valu_roofline(f32, f64, k_ms))
+        kernel = "pairs_mixed_kernel<false>"
+    else:
+        # the FP64 kernels' own work is not counted here: their line carries the reference-work
+        # rate against the FP64 peak
+        a = ref_flops / (k_ms * 1e-3) / 1e12
+        roof.update({"achieved": a, "peak": M.PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s",
+                     "frac": a / M.PEAK_FP64_VALU_TFLOPS, "basis": "reference-algorithm flops"})
+        kernel = "propagate_plain_f64" if args.precision == "fp64_plain" else "pairs_f64_kernel"
+    roof.update({"traffic": traffic_from_profile(workload, args.precision), "kernel": kernel, "kernel_ms": k_ms,
+                 "setup_kernel_ms": setup_ms, "launch_reps": args.kernel_reps,
+                 "reference_work_TFLOPs": ref_flops / (k_ms * 1e-3) / 1e12,
+                 "output_GBps": out_bytes / (k_ms * 1e-3) / 1e9})
+    counted = traffic_from_profile(workload, args.precision, "counter_flops_per_launch")
+    if counted:
+        roof["counter_flops_per_launch"] = counted
+    log(f"[bench] pair kernel {k_ms * 1e3:.1f} us, set-up {setup_ms * 1e3:.1f} us per launch "
+        f"({args.kernel_reps} back to back); {roof['achieved']:.1f} TFLOP/s = {roof['frac']:.3f} of {roof['peak']}; "
+        f"output {roof['output_GBps']:.0f} GB/s")
+
+    d2h_ms, d2h_bytes = d2h_times(props[0])
+    log(f"[bench] copy-back of one grid ({d2h_bytes / 1e6:.0f} MB) into pinned memory: {d2h_ms:.2f} ms")
+    result = {
+        "metric": "pairs/s: exact mixed-FP32 vs surrogate at max|dP|<=1e-3 vs FP64, 2048^2 grid",
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 eigenvalues and phase reduction)" if args.precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{workload}: {args.n} cosZ in [-1,1] x {args.n} E log 0.05-1000 GeV, "
+                               "12 jittered PREM shells, h=22 km, nu/nubar x NO/IO (4 grids per step)",
+                   "precision": args.precision, "pairs_per_step": pairs_per_step},
+        "roofline": roof,
+        "d2h": {"ms_per_grid": d2h_ms, "bytes_per_grid": d2h_bytes, "GBps": d2h_bytes / (d2h_ms * 1e-3) / 1e9,
+                "pairs_per_s_including_d2h": grids[0].n_pairs / ((dt / args.steps / len(grids)) + d2h_ms * 1e-3),
+                "note": "results stay in HBM in the timed steps; this is the lazy copy-back of getProbability"},
+    }
+    gpu_results = None
+    if args.precision == "mixed":
+        if not args.no_accuracy:
+            worst, rms = accuracy_vs_fp64_kernel(grids, props)
+            result["accuracy_vs_fp64_kernel"] = {"max_abs_dP": worst, "rms_dP": rms}
+            log(f"[bench] mixed vs fp64 kernel: max|dP| {worst:.3e} rms {rms:.3e}")
+        if not args.no_cpu_baseline:
+            gpu_results = []
+            for p, g in zip(props, grids):
+                p.calculateProbabilities(g.antinu)
+                gpu_results.append(p.getProbabilities().copy())
+    if not args.no_extra:
+        extra = {}
+        for prec in ("fp64", "fp64_plain"):
+            if prec == args.precision:
+                continue
+            qs = [make_prop(g, prec) for g in grids]
+            dtq = time_steps(qs, grids, 2, 1, sync)
+            extra[prec] = 2 * pairs_per_step / dtq
+            log(f"[bench] {prec}: {extra[prec]:.4e} pairs/s")
+            for q in qs:
+                q.close()
+        result["other_precisions_pairs_per_s"] = extra
+    for p in props:
+        p.close()
+    del shared
+    return result, grids, gpu_results
+
+
+def run_scan(args, sync, steps, warmup, cpu=True):
     """Batched scan: 1024 sets (seeded), each step = new hypotheses for every set + one
     set-up launch + one persistent pair launch, replayed as a captured HIP graph."""
     from cudaprob3_amd import metrics as M
@@ -129,129 +296,125 @@ def run_scan(args, sync):
     waves, gpc, tuned_ms = b.autotune()
     log(f"[bench] scan1024: {b.n_pairs} pairs, autotuned {waves} waves/workgroup x {gpc} workgroups/CU "
         f"({tuned_ms:.3f} ms pair kernel)")
-    b.compute()
-    total_ms, k_ms = b.last_ms()
+    setup_ms, k_ms = b.time_kernels(args.kernel_reps_batch)
     b.use_graph(True)
     rng = np.random.Generator(np.random.PCG64(33))
-    hyps = [np.array([W.random_params(rng) for _ in specs]) for _ in range(args.steps + args.warmup)]
+    hyps = [np.array([W.random_params(rng) for _ in specs]) for _ in range(steps + warmup)]
     # pipelined steps: the hypotheses of step k+1 are written (second descriptor array) while
     # the device runs step k; every step still computes every set with its new hypothesis
-    for k in range(args.warmup):
+    for k in range(warmup):
         b.set_params_many(hyps[k])
         b.compute_async()
     b.wait()
     sync()
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        b.set_params_many(hyps[args.warmup + k])
+    for k in range(steps):
+        b.set_params_many(hyps[warmup + k])
         b.compute_async()
     b.wait()
     sync()
     dt = time.perf_counter() - t0
-    value = args.steps * b.n_pairs / dt
-    log(f"[bench] {args.steps} scan steps in {dt:.4f} s -> {value:.4e} pairs/s")
-    flops = 0.0
+    value = steps * b.n_pairs / dt
+    log(f"[bench] scan1024: {steps} steps in {dt:.4f} s -> {value:.4e} pairs/s; pair kernel {k_ms:.3f} ms, "
+        f"set-up {setup_ms:.3f} ms per launch")
+    f32 = f64 = ref = 0.0
     for h in b.hosts:
-        flops += M.grid_flops(h.maxlayers, h.n_energies)
-    achieved = flops / (k_ms * 1e-3) / 1e12
+        a, c = M.mixed_grid_flops(h.maxlayers, h.n_energies)
+        f32, f64 = f32 + a, f64 + c
+        ref += M.grid_flops(h.maxlayers, h.n_energies)
     out_bytes = 36 * b.n_pairs
+    roof = {"bound": "valu", **M.valu_roofline(f32, f64, k_ms),
+            "traffic": traffic_from_profile("scan1024", "mixed"), "kernel": f"batch_pairs<{waves}, false>",
+            "kernel_ms": k_ms, "setup_kernel_ms": setup_ms, "launch_reps": args.kernel_reps_batch,
+            "reference_work_TFLOPs": ref / (k_ms * 1e-3) / 1e12, "output_GBps": out_bytes / (k_ms * 1e-3) / 1e9}
     result = {
         "metric": "pairs/s: batched parameter scan, mixed FP32/FP64, one persistent launch",
-        "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": dt / steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 (f64 eigenvalues and phase reduction)", "data": "synthetic",
         "config": {"workload": "scan1024: 1024 parameter sets, cosZ 64-512 x E 64-1024 each, 4-50 shell "
                                "random Earths, new hypotheses every step, HIP-graph replay, steps pipelined "
                                "(host writes step k+1 while the device runs step k)",
                    "pairs_per_step": b.n_pairs, "autotuned": {"waves_per_group": waves, "groups_per_cu": gpc}},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": M.PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / M.PEAK_FP32_VALU_TFLOPS, "traffic": traffic_from_profile("scan1024", "mixed"),
-                     "kernel": "batch_pairs", "kernel_ms": k_ms, "all_kernels_ms": total_ms,
-                     "algorithmic_flops_per_launch": flops, "output_GBps": out_bytes / (k_ms * 1e-3) / 1e9},
+        "roofline": roof,
     }
-    if not args.no_cpu_baseline:
-        import oracle
-        threads = min(16, os.cpu_count() or 1)
-        kind = "reference" if oracle.ref_available() else "port"
-        fn = oracle.ref_propagate if kind == "reference" else oracle.propagate
+    if cpu:
+        threads = host_threads()
+        kind, fn = reference_fn()
         t0 = time.perf_counter()
         pairs = 0
         for s in b.sets[:64]:
             fn(s.antinu, s.cosines, s.energies, s.radii, s.rhos, s.h_km, s.params, threads=threads)
             pairs += len(s.cosines) * len(s.energies)
         cdt = time.perf_counter() - t0
-        result["cpu_baseline"] = {"value": pairs / cdt, "unit": "pairs/s", "cores": threads, "kind": kind,
+        result["cpu_baseline"] = {"value": pairs / cdt, "unit": "pairs/s", **cpu_fields(threads), "kind": kind,
                                   "sample": f"the first 64 of the 1024 sets ({pairs} pairs), {cdt:.1f} s"}
-        log(f"[bench] CPU baseline {pairs / cdt:.4e} pairs/s")
+        log(f"[bench] scan1024 CPU baseline {pairs / cdt:.4e} pairs/s ({threads} threads)")
     b.close()
     return result
 
 
-def run_mcmc(args, sync):
+def run_mcmc(args, sync, steps, warmup, binned_steps, cpu=True):
     """Chain of sequential proposals: each step re-propagates the 8192 x 8192 grid on the
     100-shell Earth (mixed precision) and reduces (P - D)^2 on the device."""
     from cudaprob3_amd import metrics as M
     from cudaprob3_amd import workloads as W
     from cudaprob3_amd.mcmc import Chain
 
-    grid = W.fine100_grid(args.n if args.n != 2048 else 8192)
+    grid = W.fine100_grid(8192)
     chain = Chain(grid, W.NUFIT, sigma_p=0.1, seed=5)
-    log(f"[bench] mcmc: {chain.n_pairs} pairs per step, 100 shells")
-    chain.run(args.warmup)
+    log(f"[bench] mcmc8192: {chain.n_pairs} pairs per step, 100 shells")
+    chain.run(warmup)
+    setup_ms, k_ms = chain.prop.time_kernels(grid.antinu, args.kernel_reps_batch, data_ptr=chain.data.data_ptr())
     chain.stats.step_ms.clear()
     sync()
     t0 = time.perf_counter()
-    st = chain.run(args.steps)
+    st = chain.run(steps)
     sync()
     dt = time.perf_counter() - t0
-    value = args.steps * chain.n_pairs / dt
+    value = steps * chain.n_pairs / dt
     p50, p99 = st.percentile(50), st.percentile(99)
-    chain.prop.set_timing(True)
-    chain.prop.calculateProbabilities(grid.antinu)
-    total_ms, k_ms = chain.prop.last_compute_times()
     ml = M.maxlayers_of(grid.cosines, grid.radii, grid.rhos)
-    flops = M.grid_flops(ml, len(grid.energies))
-    achieved = flops / (k_ms * 1e-3) / 1e12
-    log(f"[bench] {args.steps} steps in {dt:.3f} s -> {value:.4e} pairs/s; step p50 {p50:.3f} ms p99 {p99:.3f} ms; "
+    f32, f64 = M.mixed_grid_flops(ml, len(grid.energies), likelihood=True)
+    ref = M.grid_flops(ml, len(grid.energies))
+    log(f"[bench] mcmc8192: {steps} steps in {dt:.3f} s -> {value:.4e} pairs/s; step p50 {p50:.3f} ms p99 {p99:.3f} ms; "
         f"pair kernel {k_ms:.3f} ms; accepted {st.accepted}/{st.steps}")
     result = {
         "metric": "pairs/s: Markov-chain steps re-evaluating the full grid (mixed FP32/FP64)",
-        "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": dt / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 (f64 eigenvalues and phase reduction)", "data": "synthetic",
         "config": {"workload": f"mcmc8192: {len(grid.cosines)} cosZ in [-1,1] x {len(grid.energies)} E log 0.1-1000 GeV, "
                                "100 equal PREM shells, Gaussian proposals (1% of range), device-side likelihood",
                    "pairs_per_step": chain.n_pairs},
         "latency_ms": {"p50": p50, "p99": p99, "mean": float(np.mean(st.step_ms))},
         "acceptance": st.accepted / max(st.steps, 1),
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": M.PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / M.PEAK_FP32_VALU_TFLOPS, "traffic": traffic_from_profile("mcmc8192", "mixed"),
-                     "kernel": "pairs_mixed_kernel",
-                     "kernel_ms": k_ms, "all_kernels_ms": total_ms, "algorithmic_flops_per_launch": flops},
+        "roofline": {"bound": "valu", **M.valu_roofline(f32, f64, k_ms),
+                     "traffic": traffic_from_profile("mcmc8192", "mixed"), "kernel": "pairs_mixed_kernel<true>",
+                     "kernel_ms": k_ms, "setup_kernel_ms": setup_ms, "launch_reps": args.kernel_reps_batch,
+                     "reference_work_TFLOPs": ref / (k_ms * 1e-3) / 1e12},
     }
     chain.close()
-    if not args.no_cpu_baseline:
+    if cpu:
         # one Markov-chain step's propagation on a sample of the grid: every 16th cosine row
         # (512 rows from -1 to 1, so the deep 100-shell paths are in proportion) x all energies
-        import oracle
-        threads = min(16, os.cpu_count() or 1)
-        kind = "reference" if oracle.ref_available() else "port"
-        fn = oracle.ref_propagate if kind == "reference" else oracle.propagate
+        threads = host_threads()
+        kind, fn = reference_fn()
         rows = grid.cosines[::16]
         t0 = time.perf_counter()
         fn(grid.antinu, rows, grid.energies, grid.radii, grid.rhos, grid.h_km, grid.params, threads=threads)
         cdt = time.perf_counter() - t0
         pairs = len(rows) * len(grid.energies)
-        result["cpu_baseline"] = {"value": pairs / cdt, "unit": "pairs/s", "cores": threads, "kind": kind,
+        result["cpu_baseline"] = {"value": pairs / cdt, "unit": "pairs/s", **cpu_fields(threads), "kind": kind,
                                   "sample": f"every 16th cosine row x all {len(grid.energies)} energies "
                                             f"({pairs} pairs, 100 shells), {cdt:.1f} s"}
-        log(f"[bench] CPU baseline {pairs / cdt:.4e} pairs/s")
-    if args.binned_chain_steps > 0:
-        result["binned_chains"] = binned_chain_comparison(args, sync, grid)
+        log(f"[bench] mcmc8192 CPU baseline {pairs / cdt:.4e} pairs/s ({threads} threads)")
+    if binned_steps > 0:
+        result["binned_chains"] = binned_chain_comparison(binned_steps, sync, grid)
     return result
 
 
-def binned_chain_comparison(args, sync, grid):
+def binned_chain_comparison(binned_steps, sync, grid):
     """The same chain on bin-averaged probabilities (the grid's energies as bin centres): the
     exact kernel alone (16 Gauss-Legendre nodes per bin) against the surrogate network trained
     for the 100-shell Earth plus its exact fallback."""
@@ -272,7 +435,7 @@ def binned_chain_comparison(args, sync, grid):
         ch.stats = type(ch.stats)()
         sync()
         t0 = time.perf_counter()
-        st = ch.run(args.binned_chain_steps)
+        st = ch.run(binned_steps)
         sync()
         dt = time.perf_counter() - t0
         out[method] = {"steps": st.steps, "p50_ms": st.percentile(50), "p99_ms": st.percentile(99),
@@ -293,12 +456,10 @@ def binned_chain_comparison(args, sync, grid):
 def surrogate_cpu_baseline(args):
     """Bin averages by the reference CpuPropagator<double>: one hypothesis, every 16th cosine row
     x all bins, 16 Gauss-Legendre nodes each (the exact method's work on the CPU)."""
-    import oracle
     from cudaprob3_amd import surrogate as S
     from cudaprob3_amd import workloads as W
-    threads = min(16, os.cpu_count() or 1)
-    kind = "reference" if oracle.ref_available() else "port"
-    fn = oracle.ref_propagate if kind == "reference" else oracle.propagate
+    threads = host_threads()
+    kind, fn = reference_fn()
     cos = np.linspace(-1, 1, args.n)[::16]
     E, _ = S.gl_energies(np.geomspace(0.05, 1000, args.n))
     radii, rhos = W.jitter12_model()
@@ -307,7 +468,7 @@ def surrogate_cpu_baseline(args):
     dt = time.perf_counter() - t0
     bins = len(cos) * args.n
     log(f"[bench] CPU baseline {bins / dt:.4e} bins/s")
-    return {"value": bins / dt, "unit": "bins/s", "cores": threads, "kind": kind,
+    return {"value": bins / dt, "unit": "bins/s", **cpu_fields(threads), "kind": kind,
             "sample": f"one hypothesis, every 16th cosine row x {args.n} bins x 16 nodes "
                       f"({len(cos) * len(E)} pairs), {dt:.1f} s"}
 
@@ -352,9 +513,11 @@ def surrogate_comparison(args, sync):
     mlp_ms = e0.elapsed_time(e1) / reps
     rows = slots * n * n
     mlp_tflops = rows * M.MLP_FLOPS_PER_ROW / (mlp_ms * 1e-3) / 1e12
+    mlp_exec_tflops = rows * M.MLP_EXECUTED_FLOPS_PER_ROW / (mlp_ms * 1e-3) / 1e12
     del y_t
-    log(f"[bench] network kernel: {rows / (mlp_ms * 1e-3):.4e} rows/s, {mlp_tflops:.0f} TFLOP/s "
-        f"({mlp_tflops / M.PEAK_BF16_MFMA_TFLOPS:.3f} of the BF16 MFMA peak)")
+    log(f"[bench] network kernel: {rows / (mlp_ms * 1e-3):.4e} rows/s, {mlp_tflops:.0f} TFLOP/s algorithmic "
+        f"({mlp_tflops / M.PEAK_BF16_MFMA_TFLOPS:.3f} of the BF16 MFMA peak), {mlp_exec_tflops:.0f} executed "
+        f"(two-term products)")
 
     ev = S.GridEvaluator(net, emap, cos, centers, tol=1e-3, safety=1.5, slots=slots)
     ex = S.ExactBinnedGrid(torch, cos, centers, slots=slots)
@@ -451,6 +614,8 @@ def surrogate_comparison(args, sync):
                              "achieved": mlp_tflops, "peak": M.PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
                              "frac": mlp_tflops / M.PEAK_BF16_MFMA_TFLOPS, "kernel_ms": mlp_ms,
                              "rows_per_launch": rows, "algorithmic_flops_per_row": M.MLP_FLOPS_PER_ROW,
+                             "executed_flops_per_row": M.MLP_EXECUTED_FLOPS_PER_ROW,
+                             "executed_TFLOPs": mlp_exec_tflops,
                              "rows_per_s": rows / (mlp_ms * 1e-3)},
     }
 
@@ -461,18 +626,26 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="jitter12", choices=["jitter12", "scan1024", "mcmc8192", "surrogate2048"],
-                    help="jitter12: the 2048^2 grid, four variants per step (plus the surrogate comparison); "
-                         "scan1024: 1024 parameter sets with uneven grids and Earth models in one persistent launch "
-                         "per step; mcmc8192: Markov-chain steps over an 8192^2 grid on a 100-shell Earth; "
-                         "surrogate2048: the surrogate comparison alone")
+                    help="jitter12: the 2048^2 grid, four variants per step, with the scan1024, mcmc8192 and "
+                         "surrogate lines as sub-objects; scan1024: 1024 parameter sets with uneven grids and "
+                         "Earth models in one persistent launch per step; mcmc8192: Markov-chain steps over an "
+                         "8192^2 grid on a 100-shell Earth; surrogate2048: the surrogate comparison alone")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64", "fp64_plain"])
-    ap.add_argument("--n", type=int, default=2048, help="grid edge (cosines = energies = n)")
+    ap.add_argument("--n", type=int, default=2048, help="grid edge of the jitter12 and surrogate workloads")
+    ap.add_argument("--kernel-reps", type=int, default=50,
+                    help="back-to-back launches per variant for the kernel durations of the jitter12 line")
+    ap.add_argument("--kernel-reps-batch", type=int, default=5,
+                    help="back-to-back launches for the kernel durations of the scan1024 and mcmc8192 lines")
+    ap.add_argument("--sub-steps", type=int, default=20, help="timed steps of the scan1024 / mcmc8192 sub-objects")
+    ap.add_argument("--sub-warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other precisions' throughput")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate-vs-exact comparison")
+    ap.add_argument("--no-scan", action="store_true", help="jitter12: skip the scan1024 sub-object")
+    ap.add_argument("--no-mcmc", action="store_true", help="jitter12: skip the mcmc8192 sub-object")
     ap.add_argument("--surrogate-hypotheses", type=int, default=256)
-    ap.add_argument("--binned-chain-steps", type=int, default=20,
+    ap.add_argument("--binned-chain-steps", type=int, default=10,
                     help="mcmc8192: steps of the binned exact and surrogate chains (0: skip)")
     ap.add_argument("--surrogate-validate", type=int, default=64,
                     help="hypotheses whose every bin is checked against the exact binned kernel")
@@ -487,14 +660,13 @@ def main():
         sys.exit(2)
 
     import torch
-    from cudaprob3_amd import metrics as M
-    from cudaprob3_amd import workloads as W
 
     if not torch.cuda.is_available():
         print("bench.py needs a GPU (cuda:0)", file=sys.stderr)
         sys.exit(1)
     torch.cuda.set_device(0)
     sync = torch.cuda.synchronize
+    t_start = time.perf_counter()
     if args.copy_calibration_mib:
         src = torch.ones(args.copy_calibration_mib * (1 << 18), dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -505,13 +677,13 @@ def main():
         del src, dst
 
     if args.workload == "scan1024":
-        print(json.dumps(run_scan(args, sync)), flush=True)
+        print(json.dumps(run_scan(args, sync, args.steps, args.warmup, cpu=not args.no_cpu_baseline)), flush=True)
         return
     if args.workload == "mcmc8192":
-        print(json.dumps(run_mcmc(args, sync)), flush=True)
+        print(json.dumps(run_mcmc(args, sync, args.steps, args.warmup, args.binned_chain_steps,
+                                  cpu=not args.no_cpu_baseline)), flush=True)
         return
     if args.workload == "surrogate2048":
-        t0 = time.perf_counter()
         sur = surrogate_comparison(args, sync)
         bins = args.surrogate_hypotheses * args.n * args.n
         print(json.dumps({
@@ -523,100 +695,29 @@ def main():
             "config": {"workload": "surrogate2048: " + sur["workload"]},
             "roofline": sur["network_roofline"], "surrogate": sur,
             "cpu_baseline": None if args.no_cpu_baseline else surrogate_cpu_baseline(args),
-            "wall_s": time.perf_counter() - t0}), flush=True)
+            "wall_s": time.perf_counter() - t_start}), flush=True)
         return
 
-    grids = W.jitter12_variants(args.n)
-    workload = f"jitter12_{args.n}"
-    pairs_per_step = sum(g.n_pairs for g in grids)
-    log(f"[bench] workload {workload}: {len(grids)} variants x {grids[0].n_pairs} pairs, precision {args.precision}")
-
-    # the four propagators share one stream (default priority), so their kernels run one after
-    # the other and every kernel duration is its own: with one stream per propagator the steps
-    # are 5 % shorter, but the pair kernels of different variants overlap and their durations
-    # (HIP events, rocprof) then include each other's work
-    props = [make_prop(g, args.precision) for g in grids]
-    shared = torch.cuda.Stream()
-    for p in props:
-        p.set_stream(shared.cuda_stream)
-    dt = time_steps(props, grids, args.steps, args.warmup, sync)
-    value = args.steps * pairs_per_step / dt
-    log(f"[bench] {args.steps} steps in {dt:.4f} s -> {value:.4e} pairs/s")
-
-    k_ms, total_ms = kernel_ms(props, grids, reps=3)
-    ml = M.maxlayers_of(grids[0].cosines, grids[0].radii, grids[0].rhos)
-    flops = M.grid_flops(ml, len(grids[0].energies))
-    achieved = flops / (k_ms * 1e-3) / 1e12
-    peak = M.PEAK_FP32_VALU_TFLOPS if args.precision == "mixed" else M.PEAK_FP64_VALU_TFLOPS
-    out_bytes = (4 if args.precision == "mixed" else 8) * 9 * grids[0].n_pairs
-    log(f"[bench] per-pair kernel {k_ms:.4f} ms, all kernels {total_ms:.4f} ms per launch; "
-        f"{flops / 1e9:.2f} GFLOP algorithmic -> {achieved:.1f} TFLOP/s; output {out_bytes / (k_ms * 1e-3) / 1e9:.1f} GB/s")
-
-    result = {
-        "metric": "pairs/s: exact mixed-FP32 vs surrogate at max|dP|<=1e-3 vs FP64, 2048^2 grid",
-        "value": value,
-        "unit": "pairs/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 (f64 eigenvalues and phase reduction)" if args.precision == "mixed" else "f64",
-        "data": "synthetic",
-        "config": {"workload": f"{workload}: {args.n} cosZ in [-1,1] x {args.n} E log 0.05-1000 GeV, "
-                               "12 jittered PREM shells, h=22 km, nu/nubar x NO/IO (4 grids per step)",
-                   "precision": args.precision, "pairs_per_step": pairs_per_step},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak,
-                     "traffic": traffic_from_profile(workload, args.precision),
-                     "kernel": "pairs_kernel", "kernel_ms": k_ms, "all_kernels_ms": total_ms,
-                     "algorithmic_flops_per_launch": flops,
-                     "output_GBps": out_bytes / (k_ms * 1e-3) / 1e9},
-    }
-    # what the hardware executed (rocprof instruction counters of the same kernel, committed under
-    # profiles/), over the live kernel time: the utilisation figure, where `frac` above counts the
-    # reference algorithm's work
-    executed = traffic_from_profile(workload, args.precision, "executed_gflop")
-    if executed:
-        ex_tflops = executed * 1e9 / (k_ms * 1e-3) / 1e12
-        result["roofline"].update({"executed_flops_per_launch": executed * 1e9, "executed_TFLOPs": ex_tflops,
-                                   "executed_frac_of_nonpacked_fp32_peak": ex_tflops / M.PEAK_FP32_NONPACKED_TFLOPS})
-
-    if not args.no_accuracy and args.precision == "mixed":
-        worst, rms = accuracy(grids, props)
-        result["accuracy_vs_fp64"] = {"max_abs_dP": worst, "rms_dP": rms}
-        log(f"[bench] mixed vs fp64: max|dP| {worst:.3e} rms {rms:.3e}")
-
-    if not args.no_extra:
-        extra = {}
-        for prec in ("fp64", "fp64_plain"):
-            if prec == args.precision:
-                continue
-            qs = [make_prop(g, prec) for g in grids]
-            steps = 2
-            dtq = time_steps(qs, grids, steps, 1, sync)
-            extra[prec] = steps * pairs_per_step / dtq
-            log(f"[bench] {prec}: {extra[prec]:.4e} pairs/s")
-            for q in qs:
-                q.close()
-        result["other_precisions_pairs_per_s"] = extra
-
-    if not args.no_surrogate and args.precision == "mixed":
-        for p in props:
-            p.close()
-        props = []
-        result["surrogate"] = surrogate_comparison(args, sync)
-
+    result, grids, gpu_results = run_jitter12(args, sync)
+    if args.precision == "mixed":
+        if not args.no_surrogate:
+            result["surrogate"] = surrogate_comparison(args, sync)
+        if not args.no_scan:
+            result["scan1024"] = run_scan(args, sync, args.sub_steps, args.sub_warmup, cpu=not args.no_cpu_baseline)
+        if not args.no_mcmc:
+            result["mcmc8192"] = run_mcmc(args, sync, args.sub_steps, args.sub_warmup, args.binned_chain_steps,
+                                          cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         log(f"[bench] CPU baseline with {threads} threads ...")
-        result["cpu_baseline"] = cpu_baseline(grids, threads)
-        log(f"[bench] CPU baseline {result['cpu_baseline']['value']:.4e} pairs/s")
-    for p in props:
-        p.close()
-    del shared
+        base, acc = cpu_baseline_and_accuracy(grids, gpu_results, threads)
+        if acc is not None:
+            result["accuracy_vs_reference_fp64"] = acc
+            log(f"[bench] mixed vs the reference FP64 CPU path, every pair: max|dP| {acc['max_abs_dP']:.3e} "
+                f"rms {acc['rms_dP']:.3e}")
+        result["cpu_baseline"] = base
+        log(f"[bench] CPU baseline {base['value']:.4e} pairs/s")
+    result["wall_s"] = time.perf_counter() - t_start
     print(json.dumps(result), flush=True)
 
 

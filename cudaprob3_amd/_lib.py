@@ -42,6 +42,7 @@ SIGNATURES = {
     "cp3h_compute_async": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double]),
     "cp3h_wait": (ctypes.c_int, [_handle]),
     "cp3h_compute": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double]),
+    "cp3h_fetch_results": (ctypes.c_int, [_handle, ctypes.POINTER(ctypes.c_void_p), _c_int_p]),
     "cp3h_copy_results_f64": (ctypes.c_int, [_handle, _c_double_p, ctypes.c_size_t]),
     "cp3h_copy_results_f32": (ctypes.c_int, [_handle, _c_float_p, ctypes.c_size_t]),
     "cp3h_device_results": (ctypes.c_int, [_handle, ctypes.POINTER(ctypes.c_void_p), _c_int_p]),
@@ -52,6 +53,8 @@ SIGNATURES = {
     "cp3h_compute_sqdiff_f32": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_void_p, _c_double_p]),
     "cp3h_last_compute_ms": (ctypes.c_int, [_handle, _c_float_p, _c_float_p]),
     "cp3h_set_timing": (ctypes.c_int, [_handle, ctypes.c_int]),
+    "cp3h_time_kernels": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_int,
+                                         _c_float_p, _c_float_p]),
     "cp3h_last_error": (ctypes.c_char_p, []),
     "cp3h_propagate_mixed_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                                     ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
@@ -73,6 +76,7 @@ SIGNATURES = {
     "cp3h_batch_wait": (ctypes.c_int, [_handle]),
     "cp3h_batch_compute": (ctypes.c_int, [_handle]),
     "cp3h_batch_last_ms": (ctypes.c_int, [_handle, _c_float_p, _c_float_p]),
+    "cp3h_batch_time_kernels": (ctypes.c_int, [_handle, ctypes.c_int, _c_float_p, _c_float_p]),
     "cp3h_batch_copy_results_f32": (ctypes.c_int, [_handle, ctypes.c_int, _c_float_p, ctypes.c_size_t]),
     "cp3h_batch_device_results": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "cp3h_batch_result_strides": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),

@@ -191,6 +191,13 @@ class BatchPropagator:
         _lib.check(self._lib.cp3h_batch_last_ms(self._h, ctypes.byref(t), ctypes.byref(k)))
         return t.value, k.value
 
+    def time_kernels(self, reps: int) -> tuple[float, float]:
+        """(set-up, pair) GPU milliseconds per launch, each kernel launched `reps` times back to back
+        between two HIP events (cp3h_batch_time_kernels), on the pending hypotheses."""
+        su, pa = ctypes.c_float(), ctypes.c_float()
+        _lib.check(self._lib.cp3h_batch_time_kernels(self._h, int(reps), ctypes.byref(su), ctypes.byref(pa)))
+        return su.value, pa.value
+
     def results(self, i: int) -> np.ndarray:
         """P of set i as float32 [t, ic, ie] (t = in*3+out)."""
         h = self.hosts[i]

@@ -153,6 +153,12 @@ __global__ __launch_bounds__(kWaves * kWave) void batch_pairs(BatchDesc b) {
     }
 }
 
+// Resets the work-queue counters of batch_pairs (what batch_setup also does), so that the pair
+// kernel can be launched again without a set-up in between (back-to-back timing).
+__global__ void reset_queues(unsigned long long* next_item) {
+    if (threadIdx.x < kXcds) next_item[threadIdx.x * kQueueStride] = 0;
+}
+
 struct BinWeights {
     float w[kMaxBinNodes];
 };
@@ -187,6 +193,11 @@ hipError_t launch_bin_average(const float* P, int n_c, int n_e, int row, long lo
     if (total == 0) return hipSuccess;
     const unsigned blocks = unsigned(std::min<long long>((total + kBlock - 1) / kBlock, 65535));
     hipLaunchKernelGGL(bin_average, dim3(blocks), dim3(kBlock), 0, s, P, n_c, n_e, row, plane, nodes, w, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_queues(unsigned long long* next_item, hipStream_t s) {
+    hipLaunchKernelGGL(reset_queues, dim3(1), dim3(kWave), 0, s, next_item);
     return hipGetLastError();
 }
 

@@ -173,6 +173,8 @@ hipError_t launch_batch_setup(const BatchDesc& b, int blocks, hipStream_t s);
 // b.bin_nodes > 0 each set's energies are bin_nodes quadrature nodes per bin and only the
 // weighted bin averages are stored.
 hipError_t launch_batch_pairs(const BatchDesc& b, int blocks, int waves, hipStream_t s);
+// Zeroes the pair kernel's work-queue counters (kXcds of them, kQueueStride apart).
+hipError_t launch_reset_queues(unsigned long long* next_item, hipStream_t s);
 
 // Energy-bin average of one set's results (P(t, ic, ie) at P[t * plane + ic * row + ie]) whose
 // n_e energies are `nodes` quadrature nodes per bin: out[9][n_c][n_e / nodes].

@@ -26,8 +26,10 @@ struct cp3h_propagator {
     hipStream_t stream = nullptr;      // the stream work is enqueued on
     hipStream_t own_stream = nullptr;  // created and destroyed by the handle
     hipEvent_t ev_start = nullptr, ev_pairs = nullptr, ev_stop = nullptr;
+    void* h_results = nullptr;         // pinned host copy of the results (cp3h_fetch_results)
     double* d_partials = nullptr;      // reduction scratch (cp3h_sqdiff_results_f32)
     double* d_like = nullptr;          // per-workgroup sums of the likelihood kernel
+    unsigned like_blocks = 0;          // entries of d_like
     double* h_scalar = nullptr;        // pinned landing slot for a device scalar
     double* d_cos = nullptr;
     double* d_energies = nullptr;
@@ -148,6 +150,7 @@ int cp3h_destroy(cp3h_propagator* h) {
     free_dev(h->d_partials);
     free_dev(h->d_like);
     if (h->h_scalar) (void)hipHostFree(h->h_scalar);
+    if (h->h_results) (void)hipHostFree(h->h_results);
     if (h->ev_start) (void)hipEventDestroy(h->ev_start);
     if (h->ev_pairs) (void)hipEventDestroy(h->ev_pairs);
     if (h->ev_stop) (void)hipEventDestroy(h->ev_stop);
@@ -273,7 +276,12 @@ int cp3h_set_params_f32(cp3h_propagator* h, const float* U18, const float* DM9) 
     return CP3H_OK;
 }
 
-int cp3h_compute_async(cp3h_propagator* h, int antineutrino, double production_height_cm) {
+}  // extern "C"
+
+namespace {
+
+// Checks that a computation can be launched and returns its grid description.
+int ready_to_compute(cp3h_propagator* h, int antineutrino, double production_height_cm, cp3::GridDesc& g) {
     if (int rc = check_handle(h)) return rc;
     if (antineutrino != 0 && antineutrino != 1) return fail(CP3H_ERR_ARGUMENT, "neutrino type must be 0 or 1");
     if (!h->has_energies) return fail(CP3H_ERR_STATE, "calculateProbabilities: energy list was not set");
@@ -286,31 +294,98 @@ int cp3h_compute_async(cp3h_propagator* h, int antineutrino, double production_h
             return fail(CP3H_ERR_STATE, "maxlayers exceeds the number of shells of the density model");
     CP3H_HIP(hipSetDevice(h->device));
     h->params.antinu = antineutrino;
-    const cp3::GridDesc g{h->d_cos, h->d_energies, h->d_maxlayers, h->d_radii, h->d_rhos,
-                          h->n_c, h->n_e, h->rows, production_height_cm};
-    if (h->timing) CP3H_HIP(hipEventRecord(h->ev_start, h->stream));
-    if (h->n_c == 0 || h->n_e == 0) {          // empty grid: nothing to launch
-        if (h->timing) CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
-    } else if (h->precision == CP3H_FP64_PLAIN) {
-        if (h->timing) CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
-        CP3H_HIP(cp3::launch_plain_f64(g, h->params, static_cast<double*>(h->d_out), h->stream));
+    g = cp3::GridDesc{h->d_cos, h->d_energies, h->d_maxlayers, h->d_radii, h->d_rhos,
+                      h->n_c, h->n_e, h->rows, production_height_cm};
+    return CP3H_OK;
+}
+
+// The set-up kernels of a computation (none for CP3H_FP64_PLAIN): layer lengths and the
+// per-(shell, energy) spectrum table.
+int enqueue_setup(cp3h_propagator* h, const cp3::GridDesc& g) {
+    if (h->n_c == 0 || h->n_e == 0 || h->precision == CP3H_FP64_PLAIN) return CP3H_OK;
+    if (h->precision == CP3H_MIXED) {
+        CP3H_HIP(cp3::launch_setup_mixed(g, h->params, h->ne_pad, h->d_geom, static_cast<float*>(h->d_Q), h->d_R,
+                                         h->stream));
     } else {
-        if (h->precision == CP3H_MIXED) {
-            CP3H_HIP(cp3::launch_setup_mixed(g, h->params, h->ne_pad, h->d_geom, static_cast<float*>(h->d_Q), h->d_R,
-                                             h->stream));
-            if (h->timing) CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
+        CP3H_HIP(cp3::launch_geometry(g, h->d_geom, h->stream));
+        CP3H_HIP(cp3::launch_spectrum_f64(g, h->params, h->ne_pad, static_cast<double*>(h->d_Q), h->d_R, h->stream));
+    }
+    return CP3H_OK;
+}
+
+// The per-pair kernel of a computation; with data != nullptr the likelihood variant (mixed only),
+// whose per-workgroup sums go to d_like.
+int enqueue_pairs(cp3h_propagator* h, const cp3::GridDesc& g, const float* data) {
+    if (h->n_c == 0 || h->n_e == 0) return CP3H_OK;
+    if (h->precision == CP3H_FP64_PLAIN) {
+        CP3H_HIP(cp3::launch_plain_f64(g, h->params, static_cast<double*>(h->d_out), h->stream));
+    } else if (h->precision == CP3H_MIXED) {
+        if (data)
+            CP3H_HIP(cp3::launch_pairs_mixed_sqdiff(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_R,
+                                                    wmu_f32(h->params), data, h->d_like, h->stream));
+        else
             CP3H_HIP(cp3::launch_pairs_mixed(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_R,
                                              wmu_f32(h->params), static_cast<float*>(h->d_out), h->stream));
-        } else {
-            CP3H_HIP(cp3::launch_geometry(g, h->d_geom, h->stream));
-            CP3H_HIP(cp3::launch_spectrum_f64(g, h->params, h->ne_pad, static_cast<double*>(h->d_Q), h->d_R, h->stream));
-            if (h->timing) CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
-            CP3H_HIP(cp3::launch_pairs_f64(g, h->ne_pad, h->d_geom, static_cast<const double*>(h->d_Q), h->d_R,
-                                           static_cast<double*>(h->d_out), h->stream));
-        }
+    } else {
+        CP3H_HIP(cp3::launch_pairs_f64(g, h->ne_pad, h->d_geom, static_cast<const double*>(h->d_Q), h->d_R,
+                                       static_cast<double*>(h->d_out), h->stream));
     }
+    return CP3H_OK;
+}
+
+// Scratch of the likelihood kernels (allocated on first use).
+int ensure_likelihood_scratch(cp3h_propagator* h, unsigned blocks) {
+    if (!h->d_partials) {
+        CP3H_HIP(hipMalloc(&h->d_partials, sizeof(double) * (cp3::reduce_partials_needed() + 1)));
+        CP3H_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->h_scalar), sizeof(double)));
+    }
+    if (blocks > h->like_blocks) {
+        free_dev(h->d_like);
+        CP3H_HIP(hipMalloc(&h->d_like, sizeof(double) * blocks));
+        h->like_blocks = blocks;
+    }
+    return CP3H_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cp3h_compute_async(cp3h_propagator* h, int antineutrino, double production_height_cm) {
+    cp3::GridDesc g;
+    if (int rc = ready_to_compute(h, antineutrino, production_height_cm, g)) return rc;
+    if (h->timing) CP3H_HIP(hipEventRecord(h->ev_start, h->stream));
+    if (int rc = enqueue_setup(h, g)) return rc;
+    if (h->timing) CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
+    if (int rc = enqueue_pairs(h, g, nullptr)) return rc;
     if (h->timing) CP3H_HIP(hipEventRecord(h->ev_stop, h->stream));
     h->has_timing = h->timing;
+    return CP3H_OK;
+}
+
+int cp3h_time_kernels(cp3h_propagator* h, int antineutrino, double production_height_cm, const float* data_device,
+                      int reps, float* setup_ms, float* pairs_ms) {
+    cp3::GridDesc g;
+    if (int rc = ready_to_compute(h, antineutrino, production_height_cm, g)) return rc;
+    if (reps < 1) return fail(CP3H_ERR_ARGUMENT, "reps must be >= 1");
+    if (data_device) {
+        if (h->precision != CP3H_MIXED) return fail(CP3H_ERR_STATE, "the likelihood kernel needs a CP3H_MIXED handle");
+        if (int rc = ensure_likelihood_scratch(h, cp3::likelihood_blocks(g))) return rc;
+    }
+    float ms = 0.0f;
+    CP3H_HIP(hipEventRecord(h->ev_start, h->stream));
+    for (int r = 0; r < reps; r++)
+        if (int rc = enqueue_setup(h, g)) return rc;
+    CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
+    for (int r = 0; r < reps; r++)
+        if (int rc = enqueue_pairs(h, g, data_device)) return rc;
+    CP3H_HIP(hipEventRecord(h->ev_stop, h->stream));
+    CP3H_HIP(hipEventSynchronize(h->ev_stop));
+    CP3H_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_pairs));
+    if (setup_ms) *setup_ms = ms / reps;
+    CP3H_HIP(hipEventElapsedTime(&ms, h->ev_pairs, h->ev_stop));
+    if (pairs_ms) *pairs_ms = ms / reps;
+    h->has_timing = false;       // the events no longer bracket one computation
     return CP3H_OK;
 }
 
@@ -324,37 +399,51 @@ int cp3h_compute(cp3h_propagator* h, int antineutrino, double production_height_
     return cp3h_wait(h);
 }
 
-int cp3h_copy_results_f64(cp3h_propagator* h, double* dst, size_t count) {
+int cp3h_fetch_results(cp3h_propagator* h, const void** host_ptr, int* elem_bytes) {
+    if (int rc = check_handle(h)) return rc;
+    if (!host_ptr || !elem_bytes) return fail(CP3H_ERR_ARGUMENT, "null output pointer");
+    CP3H_HIP(hipSetDevice(h->device));
+    const std::size_t bytes = std::max<std::size_t>(h->n_results(), 1) * h->out_bytes;
+    if (!h->h_results) CP3H_HIP(hipHostMalloc(&h->h_results, bytes));
+    if (h->n_results() > 0)
+        CP3H_HIP(hipMemcpyAsync(h->h_results, h->d_out, h->n_results() * h->out_bytes, hipMemcpyDeviceToHost,
+                                h->stream));
+    CP3H_HIP(hipStreamSynchronize(h->stream));
+    *host_ptr = h->h_results;
+    *elem_bytes = h->out_bytes;
+    return CP3H_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// D2H into the pinned copy, then a host copy / conversion into the caller's array.
+template <typename T>
+int copy_results(cp3h_propagator* h, T* dst, std::size_t count) {
     if (int rc = check_handle(h)) return rc;
     if (count != h->n_results()) return fail(CP3H_ERR_ARGUMENT, "result count must be 9*n_cosines*n_energies");
     if (count > 0 && !dst) return fail(CP3H_ERR_ARGUMENT, "null destination");
-    if (int rc = sync_and_select(h)) return rc;
-    if (count == 0) return CP3H_OK;
-    if (h->out_bytes == 8) {
-        CP3H_HIP(hipMemcpy(dst, h->d_out, count * 8, hipMemcpyDeviceToHost));
+    const void* src = nullptr;
+    int bytes = 0;
+    if (int rc = cp3h_fetch_results(h, &src, &bytes)) return rc;
+    if (bytes == 8) {
+        const double* s = static_cast<const double*>(src);
+        std::copy(s, s + count, dst);
     } else {
-        std::vector<float> tmp(count);
-        CP3H_HIP(hipMemcpy(tmp.data(), h->d_out, count * 4, hipMemcpyDeviceToHost));
-        for (std::size_t i = 0; i < count; i++) dst[i] = tmp[i];
+        const float* s = static_cast<const float*>(src);
+        std::copy(s, s + count, dst);
     }
     return CP3H_OK;
 }
 
-int cp3h_copy_results_f32(cp3h_propagator* h, float* dst, size_t count) {
-    if (int rc = check_handle(h)) return rc;
-    if (count != h->n_results()) return fail(CP3H_ERR_ARGUMENT, "result count must be 9*n_cosines*n_energies");
-    if (count > 0 && !dst) return fail(CP3H_ERR_ARGUMENT, "null destination");
-    if (int rc = sync_and_select(h)) return rc;
-    if (count == 0) return CP3H_OK;
-    if (h->out_bytes == 4) {
-        CP3H_HIP(hipMemcpy(dst, h->d_out, count * 4, hipMemcpyDeviceToHost));
-    } else {
-        std::vector<double> tmp(count);
-        CP3H_HIP(hipMemcpy(tmp.data(), h->d_out, count * 8, hipMemcpyDeviceToHost));
-        for (std::size_t i = 0; i < count; i++) dst[i] = float(tmp[i]);
-    }
-    return CP3H_OK;
-}
+}  // namespace
+
+extern "C" {
+
+int cp3h_copy_results_f64(cp3h_propagator* h, double* dst, size_t count) { return copy_results(h, dst, count); }
+
+int cp3h_copy_results_f32(cp3h_propagator* h, float* dst, size_t count) { return copy_results(h, dst, count); }
 
 int cp3h_device_results(cp3h_propagator* h, const void** ptr, int* elem_bytes) {
     if (int rc = check_handle(h)) return rc;
@@ -412,34 +501,19 @@ int cp3h_compute_sqdiff_f32(cp3h_propagator* h, int antineutrino, double product
                             const float* data_device, double* result) {
     if (int rc = check_handle(h)) return rc;
     if (h->precision != CP3H_MIXED) return fail(CP3H_ERR_STATE, "compute_sqdiff_f32 needs a CP3H_MIXED handle");
-    if (antineutrino != 0 && antineutrino != 1) return fail(CP3H_ERR_ARGUMENT, "neutrino type must be 0 or 1");
-    if (!h->has_energies || !h->has_cosines || !h->has_density || !h->has_params)
-        return fail(CP3H_ERR_STATE, "compute_sqdiff_f32: grid, density or oscillation parameters were not set");
-    if (!(production_height_cm >= 0.0)) return fail(CP3H_ERR_ARGUMENT, "production height must be >= 0");
     if (!result || (h->n_results() > 0 && !data_device)) return fail(CP3H_ERR_ARGUMENT, "null pointer");
-    for (int i = 0; i < h->n_c; i++)
-        if (h->maxlayers[i] > h->rows)
-            return fail(CP3H_ERR_STATE, "maxlayers exceeds the number of shells of the density model");
-    CP3H_HIP(hipSetDevice(h->device));
-    h->params.antinu = antineutrino;
-    const cp3::GridDesc g{h->d_cos, h->d_energies, h->d_maxlayers, h->d_radii, h->d_rhos,
-                          h->n_c, h->n_e, h->rows, production_height_cm};
-    if (!h->d_partials) {
-        CP3H_HIP(hipMalloc(&h->d_partials, sizeof(double) * (cp3::reduce_partials_needed() + 1)));
-        CP3H_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->h_scalar), sizeof(double)));
-    }
+    cp3::GridDesc g;
+    if (int rc = ready_to_compute(h, antineutrino, production_height_cm, g)) return rc;
     const unsigned blocks = cp3::likelihood_blocks(g);
     if (blocks == 0) {
         *result = 0.0;
         return CP3H_OK;
     }
-    if (!h->d_like) CP3H_HIP(hipMalloc(&h->d_like, sizeof(double) * blocks));
+    if (int rc = ensure_likelihood_scratch(h, blocks)) return rc;
     if (h->timing) CP3H_HIP(hipEventRecord(h->ev_start, h->stream));
-    CP3H_HIP(cp3::launch_setup_mixed(g, h->params, h->ne_pad, h->d_geom, static_cast<float*>(h->d_Q), h->d_R,
-                                     h->stream));
+    if (int rc = enqueue_setup(h, g)) return rc;
     if (h->timing) CP3H_HIP(hipEventRecord(h->ev_pairs, h->stream));
-    CP3H_HIP(cp3::launch_pairs_mixed_sqdiff(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_R,
-                                            wmu_f32(h->params), data_device, h->d_like, h->stream));
+    if (int rc = enqueue_pairs(h, g, data_device)) return rc;
     if (h->timing) CP3H_HIP(hipEventRecord(h->ev_stop, h->stream));
     h->has_timing = h->timing;
     CP3H_HIP(cp3::launch_sum_f64(h->d_like, blocks, h->d_partials, h->stream));

@@ -415,6 +415,37 @@ int cp3h_batch_last_ms(cp3h_batch* b, float* total_ms, float* pairs_ms) {
     return CP3H_OK;
 }
 
+int cp3h_batch_time_kernels(cp3h_batch* b, int reps, float* setup_ms, float* pairs_ms) {
+    if (!b) return fail(CP3H_ERR_ARGUMENT, "null batch handle");
+    if (reps < 1) return fail(CP3H_ERR_ARGUMENT, "reps must be >= 1");
+    for (int i = 0; i < b->n_sets; i++)
+        if (!b->has_grid[i] || !b->has_density[i] || !b->has_params[i])
+            return fail(CP3H_ERR_STATE, "set " + std::to_string(i) + " is missing its grid, density or parameters");
+    if (int rc = drain(b)) return rc;
+    if (b->bin_nodes == 0)
+        if (int rc = ensure_out(b)) return rc;
+    const cp3::BatchDesc d = b->desc();
+    CP3H_HIP(hipMemcpyAsync(b->d_sets, b->pinned, sizeof(cp3::BatchSet) * b->n_sets, hipMemcpyHostToDevice,
+                            b->stream));
+    CP3H_HIP(hipEventRecord(b->ev_start, b->stream));
+    for (int r = 0; r < reps; r++) CP3H_HIP(cp3::launch_batch_setup(d, b->setup_blocks(), b->stream));
+    CP3H_HIP(hipEventRecord(b->ev_pairs, b->stream));
+    for (int r = 0; r < reps; r++) {
+        CP3H_HIP(cp3::launch_reset_queues(b->d_next, b->stream));
+        if (b->item_prefix.back() > 0) CP3H_HIP(cp3::launch_batch_pairs(d, b->pair_blocks(), b->waves, b->stream));
+    }
+    CP3H_HIP(hipEventRecord(b->ev_stop, b->stream));
+    CP3H_HIP(hipEventSynchronize(b->ev_stop));
+    float ms = 0.0f;
+    CP3H_HIP(hipEventElapsedTime(&ms, b->ev_start, b->ev_pairs));
+    if (setup_ms) *setup_ms = ms / reps;
+    CP3H_HIP(hipEventElapsedTime(&ms, b->ev_pairs, b->ev_stop));
+    if (pairs_ms) *pairs_ms = ms / reps;
+    b->ever_launched = true;
+    b->last_was_graph = true;        // cp3h_batch_last_ms: the events no longer bracket one computation
+    return CP3H_OK;
+}
+
 int cp3h_batch_autotune(cp3h_batch* b, int* waves_per_group, int* groups_per_cu, float* best_ms) {
     if (!b) return fail(CP3H_ERR_ARGUMENT, "null batch handle");
     const bool graph = b->use_graph;

@@ -273,22 +273,15 @@ class HipPropagatorSingle(Propagator):
             _lib.check(self._lib.cp3h_set_maxlayers(self._h, _ptr(m, ctypes.c_int), len(m)))
 
     # ---- compute (cudapropagator.cuh:162-226) -------------------------------------------
-    def setMNSMatrix(self, theta12, theta13, theta23, dCP):
-        super().setMNSMatrix(theta12, theta13, theta23, dCP)
-        self._params_uploaded = False
-
-    def setNeutrinoMasses(self, dm12sq, dm23sq):
-        super().setNeutrinoMasses(dm12sq, dm23sq)
-        self._params_uploaded = False
-
     def calculateProbabilitiesAsync(self, type_):
+        """Like the reference (cudapropagator.cuh:191-220), every computation hands the current
+        mixing matrix and mass differences over again (the library then redoes their host
+        set-up: vacuum root order and the mixed kernels' real basis)."""
         h = self._handle()
         if not self.isSetProductionHeight:
             raise RuntimeError("HipPropagatorSingle::calculateProbabilities. production height was not set")
         self.resultsResideOnHost = False
-        if not getattr(self, "_params_uploaded", False):
-            self._upload_params(h)
-            self._params_uploaded = True
+        self._upload_params(h)
         _lib.check(self._lib.cp3h_compute_async(h, int(type_), float(self.ProductionHeightinCentimeter)))
 
     def computeSqdiff(self, type_, data_ptr: int) -> float:
@@ -298,23 +291,25 @@ class HipPropagatorSingle(Propagator):
         h = self._handle()
         if not self.isSetProductionHeight:
             raise RuntimeError("HipPropagatorSingle::calculateProbabilities. production height was not set")
-        if not getattr(self, "_params_uploaded", False):
-            self._upload_params(h)
-            self._params_uploaded = True
+        self._upload_params(h)
         r = ctypes.c_double()
         _lib.check(self._lib.cp3h_compute_sqdiff_f32(h, int(type_), float(self.ProductionHeightinCentimeter),
                                                      ctypes.c_void_p(data_ptr), ctypes.byref(r)))
         return r.value
 
     def _upload_params(self, h):
-        if self.dtype == np.float64:
-            U = np.ascontiguousarray(self.Mix_U, dtype=np.float64)
-            D = np.ascontiguousarray(self.dm, dtype=np.float64)
-            _lib.check(self._lib.cp3h_set_params_f64(h, _ptr(U, ctypes.c_double), _ptr(D, ctypes.c_double)))
-        else:
-            U = np.ascontiguousarray(self.Mix_U, dtype=np.float32)
-            D = np.ascontiguousarray(self.dm, dtype=np.float32)
-            _lib.check(self._lib.cp3h_set_params_f32(h, _ptr(U, ctypes.c_float), _ptr(D, ctypes.c_float)))
+        """Hands U and DM to the handle (cp3h_set_params_*, which redoes the host set-up derived
+        from them).  The marshalled arrays are kept while Mix_U and dm are the same objects."""
+        key = (id(self.Mix_U), id(self.dm))
+        cached = getattr(self, "_param_args", None)
+        if cached is None or cached[0] != key:
+            ctype = ctypes.c_double if self.dtype == np.float64 else ctypes.c_float
+            U = np.ascontiguousarray(self.Mix_U, dtype=self.dtype)
+            D = np.ascontiguousarray(self.dm, dtype=self.dtype)
+            fn = self._lib.cp3h_set_params_f64 if self.dtype == np.float64 else self._lib.cp3h_set_params_f32
+            cached = (key, fn, _ptr(U, ctype), _ptr(D, ctype), U, D, self.Mix_U, self.dm)
+            self._param_args = cached
+        _lib.check(cached[1](h, cached[2], cached[3]))
 
     def waitForCompletion(self):
         _lib.check(self._lib.cp3h_wait(self._handle()))
@@ -324,12 +319,17 @@ class HipPropagatorSingle(Propagator):
         self.waitForCompletion()
 
     def getResultFromDevice(self):
+        """cudapropagator.cuh:229-235: D2H into the handle's pinned host buffer (cp3h_fetch_results),
+        then a copy in FLOAT_T (the pinned buffer is overwritten by the next fetch)."""
         n = 9 * self.n_cosines * self.n_energies
-        out = np.empty(n, dtype=self.dtype)
-        if self.dtype == np.float64:
-            _lib.check(self._lib.cp3h_copy_results_f64(self._handle(), _ptr(out, ctypes.c_double), n))
+        ptr, nbytes = ctypes.c_void_p(), ctypes.c_int()
+        _lib.check(self._lib.cp3h_fetch_results(self._handle(), ctypes.byref(ptr), ctypes.byref(nbytes)))
+        ctype = ctypes.c_double if nbytes.value == 8 else ctypes.c_float
+        if n:
+            pinned = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), shape=(n,))
+            out = pinned.astype(self.dtype, copy=True)
         else:
-            _lib.check(self._lib.cp3h_copy_results_f32(self._handle(), _ptr(out, ctypes.c_float), n))
+            out = np.empty(0, dtype=self.dtype)
         self._host_result = out.reshape(9, self.n_cosines, self.n_energies)
         self.resultsResideOnHost = True
 
@@ -364,6 +364,20 @@ class HipPropagatorSingle(Propagator):
     def set_timing(self, enable: bool):
         """Record HIP events around every compute (for last_compute_times); off by default."""
         _lib.check(self._lib.cp3h_set_timing(self._handle(), int(bool(enable))))
+
+    def time_kernels(self, type_, reps: int, data_ptr: int | None = None) -> tuple[float, float]:
+        """(set-up, per-pair) GPU milliseconds per launch, each kernel launched `reps` times back to
+        back between two HIP events (cp3h_time_kernels); data_ptr times the likelihood variant."""
+        h = self._handle()
+        if not self.isSetProductionHeight:
+            raise RuntimeError("HipPropagatorSingle::calculateProbabilities. production height was not set")
+        self._upload_params(h)
+        self.resultsResideOnHost = False
+        su, pa = ctypes.c_float(), ctypes.c_float()
+        _lib.check(self._lib.cp3h_time_kernels(h, int(type_), float(self.ProductionHeightinCentimeter),
+                                               ctypes.c_void_p(data_ptr or None), int(reps), ctypes.byref(su),
+                                               ctypes.byref(pa)))
+        return su.value, pa.value
 
     def last_compute_ms(self) -> float:
         """GPU milliseconds of all kernels of the last compute."""

@@ -282,10 +282,16 @@ def train(net: SurrogateNet, data: BinnedSets, epochs: int, batch: int = 8192, l
 
 # --- error map and fallback ---------------------------------------------------------------------
 
-N_COS_CELLS = 32                 # error-map cells: 32 cosZ cells x 0.1-decade energy cells,
+N_COS_CELLS = 32                 # default error-map cells: 32 cosZ cells x 0.1-decade energy cells,
 E_CELL_DECADES = 0.1             # per hypothesis class (nu / nubar x normal / inverted ordering)
-N_E_CELLS = int(math.ceil((LOG10E_RANGE[1] - LOG10E_RANGE[0]) / E_CELL_DECADES - 1e-9))
 N_CLASSES = 4
+MIN_CELL_SAMPLES = 32            # a cell seen by fewer held-out samples is never trusted
+
+
+def n_energy_cells(decades: float) -> int:
+    """Energy cells of about `decades` each over LOG10E_RANGE (the last one absorbs the
+    remainder, so no cell is a sliver at the range's upper edge)."""
+    return max(1, int(round((LOG10E_RANGE[1] - LOG10E_RANGE[0]) / decades)))
 
 
 def hypothesis_class(params, antinu) -> int:
@@ -294,12 +300,16 @@ def hypothesis_class(params, antinu) -> int:
 
 @dataclass
 class ErrorMap:
-    max_err: np.ndarray      # [class][cos cell][energy cell]: largest held-out |dP| (any channel)
+    """max_err[class][cos cell][energy cell]: the largest held-out |dP| (any channel) in the
+    cell, +inf where fewer than MIN_CELL_SAMPLES held-out samples fell.  The cells split
+    cosZ in [-1, 1] and log10 E over LOG10E_RANGE evenly (their counts are the array's shape)."""
+    max_err: np.ndarray
 
-    @staticmethod
-    def cells(cos, log10E):
-        ic = np.clip(((np.asarray(cos) + 1.0) / 2.0 * N_COS_CELLS).astype(int), 0, N_COS_CELLS - 1)
-        ie = np.clip(((np.asarray(log10E) - LOG10E_RANGE[0]) / E_CELL_DECADES).astype(int), 0, N_E_CELLS - 1)
+    def cells(self, cos, log10E):
+        n_c, n_e = self.max_err.shape[1], self.max_err.shape[2]
+        ic = np.clip(((np.asarray(cos) + 1.0) / 2.0 * n_c).astype(int), 0, n_c - 1)
+        width = (LOG10E_RANGE[1] - LOG10E_RANGE[0]) / n_e
+        ie = np.clip(((np.asarray(log10E) - LOG10E_RANGE[0]) / width).astype(int), 0, n_e - 1)
         return ic, ie
 
     def fallback(self, k: int, cos, log10E, tol: float, safety: float) -> np.ndarray:
@@ -309,20 +319,27 @@ class ErrorMap:
         return self.max_err[k][ic[:, None], ie[None, :]] * safety > tol
 
 
-def build_error_map(net: SurrogateNet, data: BinnedSets) -> ErrorMap:
-    """Largest |dP| of the network on held-out exact data, per class and cell."""
+def build_error_map(net: SurrogateNet, data: BinnedSets, n_cos_cells: int = N_COS_CELLS,
+                    e_cell_decades: float = E_CELL_DECADES, min_samples: int = MIN_CELL_SAMPLES) -> ErrorMap:
+    """Largest |dP| of the network on held-out exact data, per class and cell; cells with
+    fewer than min_samples held-out samples get +inf (they always fall back)."""
     torch = net.torch
     Y = net.forward(data.X)
     net.sync()
     err = (Y - data.T).abs().amax(dim=1)
     X = data.X
-    ic = ((X[:, 0] + 1.0) / 2.0 * N_COS_CELLS).long().clamp(0, N_COS_CELLS - 1)
-    ie = ((X[:, 1] - LOG10E_RANGE[0]) / E_CELL_DECADES).long().clamp(0, N_E_CELLS - 1)
+    n_e = n_energy_cells(e_cell_decades)
+    width = (LOG10E_RANGE[1] - LOG10E_RANGE[0]) / n_e
+    ic = ((X[:, 0] + 1.0) / 2.0 * n_cos_cells).long().clamp(0, n_cos_cells - 1)
+    ie = ((X[:, 1] - LOG10E_RANGE[0]) / width).long().clamp(0, n_e - 1)
     k = 2 * X[:, 8].round().long() + (X[:, 7] < 0).long()
-    cell = (k * N_COS_CELLS + ic) * N_E_CELLS + ie
-    m = torch.zeros(N_CLASSES * N_COS_CELLS * N_E_CELLS, dtype=torch.float32, device=err.device)
-    m.scatter_reduce_(0, cell, err, reduce="amax")
-    return ErrorMap(m.view(N_CLASSES, N_COS_CELLS, N_E_CELLS).cpu().numpy())
+    cell = (k * n_cos_cells + ic) * n_e + ie
+    n_cells = N_CLASSES * n_cos_cells * n_e
+    m = torch.zeros(n_cells, dtype=torch.float32, device=err.device)
+    m.scatter_reduce_(0, cell, err, reduce="amax", include_self=False)
+    count = torch.bincount(cell, minlength=n_cells)
+    m[count < min_samples] = float("inf")
+    return ErrorMap(m.view(N_CLASSES, n_cos_cells, n_e).cpu().numpy())
 
 
 # --- evaluation on a fixed grid ------------------------------------------------------------------

@@ -48,6 +48,24 @@ def load(path: str = DEFAULT_PATH, max_batch: int = 8192, device: int = 0):
         return net, S.ErrorMap(z["max_err"].astype(np.float64)), json.loads(str(z["meta"]))
 
 
+def rebuild_map(path: str, held_sets: int, n_cos_cells: int, e_cell_decades: float, min_samples: int,
+                seed: int | None = None):
+    """Recomputes the error map of a saved network on `held_sets` fresh held-out hypotheses (same
+    Earth model; seed default: the training seed + 1) and rewrites the file."""
+    net, _, meta = load(path, max_batch=65536)
+    earth = meta.get("earth", "jitter12")
+    seed = meta["seed"] + 1 if seed is None else seed
+    held = S.make_dataset(n_sets=held_sets, n_cos=64, n_bins=64, seed=seed, earth=earth)
+    emap = S.build_error_map(net, held, n_cos_cells=n_cos_cells, e_cell_decades=e_cell_decades,
+                             min_samples=min_samples)
+    meta.update(earth=earth, held_out_samples=held.X.shape[0], held_out_seed=seed,
+                error_map={"cos_cells": n_cos_cells, "energy_cells": int(emap.max_err.shape[2]),
+                           "min_cell_samples": min_samples})
+    save(path, net, emap, meta)
+    net.close()
+    return emap, meta
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--samples", type=int, default=50_331_648)
@@ -58,8 +76,18 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=101)
     ap.add_argument("--earth", default="jitter12", choices=sorted(S.EARTHS))
     ap.add_argument("--out", default=None, help="default: cudaprob3_amd/data/surrogate_<earth>.npz")
+    ap.add_argument("--held-sets", type=int, default=1024, help="held-out hypotheses of the error map")
+    ap.add_argument("--cos-cells", type=int, default=S.N_COS_CELLS)
+    ap.add_argument("--e-cell-decades", type=float, default=S.E_CELL_DECADES)
+    ap.add_argument("--min-cell-samples", type=int, default=S.MIN_CELL_SAMPLES)
+    ap.add_argument("--rebuild-map", action="store_true",
+                    help="keep the saved network, recompute only its error map")
     a = ap.parse_args(argv)
     out = a.out or path_for(a.earth)
+    if a.rebuild_map:
+        _, meta = rebuild_map(out, a.held_sets, a.cos_cells, a.e_cell_decades, a.min_cell_samples)
+        print(json.dumps(meta), flush=True)
+        return
     t0 = time.time()
     lo = math.log10(max(a.min_energy, 10 ** S.LOG10E_RANGE[0]))
     n_sets = a.samples // 4096
@@ -71,11 +99,15 @@ def main(argv=None):
     hist = S.train(net, data, epochs=a.epochs, batch=a.batch, lr_max=a.lr, seed=a.seed,
                    log=lambda s: print(f"[{time.time() - t1:.0f} s] {s}", flush=True))
     del data
-    held = S.make_dataset(n_sets=1024, n_cos=64, n_bins=64, seed=a.seed + 1, earth=a.earth)
-    emap = S.build_error_map(net, held)
+    held = S.make_dataset(n_sets=a.held_sets, n_cos=64, n_bins=64, seed=a.seed + 1, earth=a.earth)
+    emap = S.build_error_map(net, held, n_cos_cells=a.cos_cells, e_cell_decades=a.e_cell_decades,
+                             min_samples=a.min_cell_samples)
     meta = dict(earth=a.earth, samples=n_sets * 4096, epochs=a.epochs, batch=a.batch, lr=a.lr,
                 min_energy_gev=10 ** lo, seed=a.seed,
-                final_loss=hist[-1] if hist else None, train_seconds=time.time() - t1, held_out_samples=held.X.shape[0])
+                final_loss=hist[-1] if hist else None, train_seconds=time.time() - t1, held_out_samples=held.X.shape[0],
+                held_out_seed=a.seed + 1,
+                error_map={"cos_cells": a.cos_cells, "energy_cells": int(emap.max_err.shape[2]),
+                           "min_cell_samples": a.min_cell_samples})
     save(out, net, emap, meta)
     print(json.dumps(meta), flush=True)
 

@@ -90,8 +90,13 @@ int cp3h_wait(cp3h_propagator* h);
 /* cudapropagator.cuh:162-165 (calculateProbabilities) = compute_async + wait. */
 int cp3h_compute(cp3h_propagator* h, int antineutrino, double production_height_cm);
 
-/* cudapropagator.cuh:229-235 (getResultFromDevice): copy all 9*n_c*n_e results (SoA)
-   to host memory, converting to the requested type. */
+/* cudapropagator.cuh:229-235 (getResultFromDevice) with the pinned result buffer of
+   cudapropagator.cuh:73: copies all 9*n_c*n_e results (SoA, the handle's element type: 8 bytes
+   for the FP64 modes, 4 for CP3H_MIXED) into a pinned host buffer the handle owns (allocated on
+   the first call) and returns it.  Valid until the next fetch or the handle's destruction. */
+int cp3h_fetch_results(cp3h_propagator* h, const void** host_ptr, int* elem_bytes);
+
+/* The same into a caller array, converting to the requested type (through the pinned buffer). */
 int cp3h_copy_results_f64(cp3h_propagator* h, double* dst, size_t count);
 int cp3h_copy_results_f32(cp3h_propagator* h, float* dst, size_t count);
 
@@ -130,6 +135,17 @@ int cp3h_last_compute_ms(cp3h_propagator* h, float* total_ms, float* pairs_ms);
    records HIP events around its kernels for cp3h_last_compute_ms; off by default (each event
    record adds a gap of several microseconds between the kernels). */
 int cp3h_set_timing(cp3h_propagator* h, int enable);
+
+/* Benchmark support (no counterpart in the reference; the reference times only wall clock,
+   example/main.cpp:130-139): launches the computation's set-up kernels `reps` times back to back,
+   then its per-pair kernel `reps` times back to back, between HIP events on the handle's stream,
+   and returns the average duration of one launch of each (setup_ms is 0 for CP3H_FP64_PLAIN,
+   which has no set-up).  Back-to-back launches hide the gaps that an event pair around a single
+   launch would include.  data_device != NULL times the likelihood variant of
+   cp3h_compute_sqdiff_f32 (CP3H_MIXED) instead of the storing kernel.  Waits for completion; the
+   results are those of one computation.  Either pointer may be NULL. */
+int cp3h_time_kernels(cp3h_propagator* h, int antineutrino, double production_height_cm, const float* data_device,
+                      int reps, float* setup_ms, float* pairs_ms);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* cp3h_last_error(void);
@@ -202,6 +218,11 @@ int cp3h_batch_compute(cp3h_batch* b);
 /* GPU time of the last computation (waits for it): all kernels, and the pair kernel alone
    (pairs_ms is -1 when the computation was a graph replay). */
 int cp3h_batch_last_ms(cp3h_batch* b, float* total_ms, float* pairs_ms);
+/* Benchmark support: the set-up kernel `reps` times back to back, then the persistent pair
+   kernel `reps` times back to back (each after a one-workgroup reset of its work queues, as the
+   set-up does), between HIP events; returns the average duration of one launch of each.  Uses
+   the pending hypotheses; waits for completion. */
+int cp3h_batch_time_kernels(cp3h_batch* b, int reps, float* setup_ms, float* pairs_ms);
 
 /* Results of set i, SoA [t][ic][ie] float, count = 9 * n_cosines[i] * n_energies[i] (copied
    densely).  On the device each cosine row starts a 128-byte line: P(t, ic, ie) is at

@@ -3,8 +3,9 @@
 //
 // Header-only C++ over the C ABI of libcp3hip.so (include/cp3hip.h): the base class does the
 // host set-up, this class forwards the results to the device handle, launches, and copies
-// the probabilities back lazily on the first getProbability after a computation, like the
-// reference.  Link with -lcp3hip.  Device failures throw std::runtime_error (the reference
+// the probabilities back lazily, into the handle's pinned host buffer, on the first
+// getProbability after a computation, like the reference (cudapropagator.cuh:73, 172-175,
+// 229-235).  Link with -lcp3hip.  Device failures throw std::runtime_error (the reference
 // prints and exits).
 #ifndef CUDAPROB3_MI355X_HIPPROPAGATOR_HPP
 #define CUDAPROB3_MI355X_HIPPROPAGATOR_HPP
@@ -54,8 +55,6 @@ inline int set_params(cp3h_propagator* h, const double* U18, const double* DM9) 
 inline int set_params(cp3h_propagator* h, const float* U18, const float* DM9) {
     return cp3h_set_params_f32(h, U18, DM9);
 }
-inline int copy_results(cp3h_propagator* h, double* dst, std::size_t n) { return cp3h_copy_results_f64(h, dst, n); }
-inline int copy_results(cp3h_propagator* h, float* dst, std::size_t n) { return cp3h_copy_results_f32(h, dst, n); }
 }  // namespace detail
 
 template <class FLOAT_T>
@@ -70,7 +69,6 @@ public:
         : Propagator<FLOAT_T>(n_cosines_, n_energies_), deviceId(id) {
         detail::check(cp3h_create(id, n_cosines_, n_energies_, int(precision), &handle),
                       "HipPropagatorSingle");
-        resultList.resize(std::size_t(n_cosines_) * std::size_t(n_energies_) * 9);
     }
 
     // cudapropagator.cuh:86: device 0.
@@ -89,7 +87,9 @@ public:
             cp3h_destroy(handle);
             handle = other.handle;
             other.handle = nullptr;
-            resultList = std::move(other.resultList);
+            hostResults = other.hostResults;
+            resultBytes = other.resultBytes;
+            other.hostResults = nullptr;
             deviceId = other.deviceId;
             resultsResideOnHost = other.resultsResideOnHost;
         }
@@ -129,7 +129,8 @@ public:
         }
         const std::uint64_t index = std::uint64_t(index_cosine) * std::uint64_t(this->n_energies) + index_energy;
         const std::uint64_t offset = std::uint64_t(t) * std::uint64_t(this->n_energies) * this->n_cosines;
-        return resultList[index + offset];
+        return resultBytes == 8 ? FLOAT_T(static_cast<const double*>(hostResults)[index + offset])
+                                : FLOAT_T(static_cast<const float*>(hostResults)[index + offset]);
     }
 
     // Extensions: device-resident results (SoA, element size 4 for Precision::Mixed, else 8)
@@ -176,10 +177,9 @@ protected:
                           "HipPropagatorSingle::setMaxlayers");
     }
 
-    // cudapropagator.cuh:229-235
+    // cudapropagator.cuh:229-235: D2H into the handle's pinned host buffer
     void getResultFromDevice() {
-        detail::check(detail::copy_results(live(), resultList.data(), resultList.size()),
-                      "HipPropagatorSingle::getProbability");
+        detail::check(cp3h_fetch_results(live(), &hostResults, &resultBytes), "HipPropagatorSingle::getProbability");
     }
 
 private:
@@ -189,7 +189,8 @@ private:
     }
 
     cp3h_propagator* handle = nullptr;
-    std::vector<FLOAT_T> resultList;
+    const void* hostResults = nullptr;   // pinned, owned by the handle; SoA [t][ic][ie]
+    int resultBytes = 8;                 // 8 (FP64 modes) or 4 (Precision::Mixed)
     int deviceId = 0;
     bool resultsResideOnHost = false;
 };

@@ -2,11 +2,12 @@
 golden vectors.
 
 Tolerances (max |dP| over all nine channels):
-  * fp64_plain, fp64 : 1e-12 against the reference / oracle FP64 CPU path, or twice the
-                       reference algorithm's own drift under a one-ulp change of libm where
-                       that is larger (only the eigenvalue-degenerate dm23 = 0 case, whose
-                       5e-9 eV^2 nudge amplifies libm rounding; see oracle.libm_sensitivity);
-  * mixed            : MIXED_TOL against the same FP64 results (FP32 bulk arithmetic,
+  * fp64_plain, fp64 : FP64_TOL = 1e-12 against the reference / oracle FP64 CPU path, with one
+                       named exception: on the golden fixture degenerate_dm23_zero (dm23 = 0,
+                       whose 5e-9 eV^2 degeneracy nudge, propagator.hpp:233-236, amplifies libm
+                       rounding) the bar is twice the reference algorithm's own drift under a
+                       one-ulp change of libm (oracle.libm_sensitivity), below 1e-11;
+  * mixed            : MIXED_TOL = 1e-5 against the same FP64 results (FP32 bulk arithmetic,
                        FP64 eigenvalues and phase reduction).
 """
 import numpy as np
@@ -22,10 +23,18 @@ FP64_TOL = 1e-12
 MIXED_TOL = 1e-5
 
 
-def tolerance(precision, oracle_mod, reference, *inputs):
+# the one input whose FP64 bar follows the reference's libm sensitivity instead of 1e-12
+LIBM_RELAXED = {"degenerate_dm23_zero"}
+
+
+def tolerance(precision, oracle_mod=None, reference=None, *inputs, name=None):
     if precision == "mixed":
         return MIXED_TOL
-    return oracle_mod.fp64_tolerance(oracle_mod.libm_sensitivity(*inputs, reference=reference))
+    if name in LIBM_RELAXED:
+        tol = oracle_mod.fp64_tolerance(oracle_mod.libm_sensitivity(*inputs, reference=reference))
+        assert tol < 1e-11
+        return tol
+    return FP64_TOL
 
 
 def run_gpu(precision, cosines, energies, radii, rhos, params, h_km, antinu, device=0):
@@ -50,7 +59,7 @@ def test_golden(oracle_mod, precision, name):
                 float(g["h_km"]), int(g["antinu"]))
     err = np.abs(P - g["P"]).max()
     tol = tolerance(precision, oracle_mod, g["P"], int(g["antinu"]), g["cosines"], g["energies"], g["radii"],
-                    g["rhos"], float(g["h_km"]), g["params"])
+                    g["rhos"], float(g["h_km"]), g["params"], name=name)
     print(f"[gpu]   max|dP| {err:.3e} (bar {tol:.1e})")
     assert err <= tol, f"{name} {precision}: max|dP| = {err:.3e}"
 
@@ -64,7 +73,8 @@ def test_prem4_256_full_grid(oracle_mod, precision, antinu):
     ref = oracle_mod.propagate(antinu, g.cosines, g.energies, g.radii, g.rhos, g.h_km, g.params)
     P = run_gpu(precision, g.cosines, g.energies, g.radii, g.rhos, g.params, g.h_km, antinu)
     err = np.abs(P - ref)
-    tol = tolerance(precision, oracle_mod, ref, antinu, g.cosines, g.energies, g.radii, g.rhos, g.h_km, g.params)
+    tol = tolerance(precision)
+    assert tol == (MIXED_TOL if precision == "mixed" else 1e-12)
     print(f"[gpu] prem4_256 {precision} antinu={antinu}: max {err.max():.3e} rms {np.sqrt((err**2).mean()):.3e}"
           f" (bar {tol:.1e})")
     assert err.max() <= tol
@@ -80,8 +90,7 @@ def test_ragged_and_empty_grids(oracle_mod, precision, n_c, n_e):
     assert P.shape == (n_c, n_e, 9)
     if n_c and n_e:
         ref = oracle_mod.propagate(0, cos, E, W.PREM12[:, 0], W.PREM12[:, 1], 22.0, W.NUFIT)
-        tol = tolerance(precision, oracle_mod, ref, 0, cos, E, W.PREM12[:, 0], W.PREM12[:, 1], 22.0, W.NUFIT)
-        assert np.abs(P - ref).max() <= tol
+        assert np.abs(P - ref).max() <= tolerance(precision)
 
 
 def test_mixed_rows_together_equal_rows_alone():
@@ -107,21 +116,34 @@ def test_mixed_rows_together_equal_rows_alone():
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_jitter12_2048_full_size(oracle_mod, precision):
-    """The 2048 x 2048 jittered 12-shell workload, all four nu/nubar x NO/IO variants:
-    strided rows against the oracle, unitarity over the full grid."""
+    """The 2048 x 2048 jittered 12-shell workload, all four nu/nubar x NO/IO variants: strided rows
+    against the oracle; for mixed, every one of the 4.2 M pairs per variant against the FP64
+    kernel (held to the oracle at 1e-12 by the fp64 case), and the directly computed entries
+    (P_ee, P_emu, P_mue, P_mumu; the other five follow from unitarity) summing to at most one
+    along each row and column; for fp64, whose nine entries are all computed, unitarity."""
     for g in W.jitter12_variants(2048):
         print(f"[gpu] jitter12_2048 {precision} antinu={g.antinu} dm32={g.params[5]}", flush=True)
         P = run_gpu(precision, g.cosines, g.energies, g.radii, g.rhos, g.params, g.h_km, g.antinu)
         rows = np.arange(0, 2048, 37)
         ref = oracle_mod.propagate(g.antinu, g.cosines[rows], g.energies, g.radii, g.rhos, g.h_km, g.params)
         err = np.abs(P[rows] - ref)
-        tol = tolerance(precision, oracle_mod, ref, g.antinu, g.cosines[rows], g.energies, g.radii, g.rhos,
-                        g.h_km, g.params)
-        print(f"[gpu]   max {err.max():.3e} rms {np.sqrt((err**2).mean()):.3e} (bar {tol:.1e})")
+        tol = tolerance(precision)
+        print(f"[gpu]   vs oracle on {len(rows)} rows: max {err.max():.3e} rms {np.sqrt((err**2).mean()):.3e} "
+              f"(bar {tol:.1e})")
         assert err.max() <= tol
         M = P.reshape(2048, 2048, 3, 3)
-        uni = max(np.abs(M.sum(axis=3) - 1).max(), np.abs(M.sum(axis=2) - 1).max())
-        assert uni <= (1e-11 if precision == "fp64" else 10 * MIXED_TOL)
+        if precision == "mixed":
+            P64 = run_gpu("fp64", g.cosines, g.energies, g.radii, g.rhos, g.params, g.h_km, g.antinu)
+            e64 = np.abs(P - P64)
+            print(f"[gpu]   vs fp64 kernel, all pairs: max {e64.max():.3e} rms {np.sqrt((e64 ** 2).mean()):.3e}")
+            assert e64.max() <= MIXED_TOL
+            del P64, e64
+            direct = M[:, :, :2, :2]
+            over = max((direct.sum(axis=3) - 1).max(), (direct.sum(axis=2) - 1).max())
+            assert over <= MIXED_TOL
+        else:
+            uni = max(np.abs(M.sum(axis=3) - 1).max(), np.abs(M.sum(axis=2) - 1).max())
+            assert uni <= 1e-11
 
 
 @pytest.mark.parametrize("antinu,inverted", [(0, False), (1, False), (0, True)])
@@ -207,8 +229,7 @@ def test_independent_handles_do_not_share_parameters(oracle_mod):
     for p, h in zip(props, hyp):
         p.waitForCompletion()
         ref = oracle_mod.propagate(0, cos, E, g.radii, g.rhos, 22.0, h)
-        tol = tolerance("fp64", oracle_mod, ref, 0, cos, E, g.radii, g.rhos, 22.0, h)
-        assert np.abs(p.getProbabilities().transpose(1, 2, 0) - ref).max() <= tol
+        assert np.abs(p.getProbabilities().transpose(1, 2, 0) - ref).max() <= tolerance("fp64")
         p.close()
 
 
@@ -229,7 +250,6 @@ def test_mixed_any_phase_convention(name):
         V = np.diag(np.exp(1j * rng.uniform(0, 6.3, 3))) @ U @ np.diag(np.exp(1j * rng.uniform(0, 6.3, 3)))
         assert np.abs(V - U).max() > 0.1
         p.Mix_U = list(np.stack([V.real, V.imag], -1).reshape(18))
-        p._params_uploaded = False
         print(f"[gpu] rephased U, {name}", flush=True)
         p.calculateProbabilities(Antineutrino if int(g["antinu"]) else Neutrino)
         err = np.abs(p.getProbabilities().transpose(1, 2, 0) - g["P"]).max()
