@@ -153,24 +153,10 @@ __device__ inline double path_length(double cz, double h_cm) {
            kREarthCm * cz;
 }
 
-// C = A B, with the summation order of math.hpp:50-64.
-__device__ inline void mat_mul(const Cd A[3][3], const Cd B[3][3], Cd C[3][3]) {
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            Cd c{0.0, 0.0};
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                c.re += A[i][k].re * B[k][j].re - A[i][k].im * B[k][j].im;
-                c.im += A[i][k].im * B[k][j].re + A[i][k].re * B[k][j].im;
-            }
-            C[i][j] = c;
-        }
-}
-
-// One constant-density layer's flavour-basis amplitude (physics.hpp:434-531) with the
-// AXFAC table read from LDS.
+// One constant-density layer's flavour-basis amplitude (physics.hpp:434-531) with the AXFAC
+// table read from LDS.  Each X[i][j] = sum_k e^{i phi_k} P_k[i][j] is added into every A[n][m]
+// as soon as it is formed, so X is never held whole; every A[n][m] still receives its terms
+// in the reference's (i, j) order, which keeps its rounding.
 __device__ inline void layer_amplitude(const OscParams& p, const double* __restrict__ ax, double E,
                                        double rho, double Lkm, Cd A[3][3]) {
     double mm[3][3], mv[3][3];
@@ -184,7 +170,10 @@ __device__ inline void layer_amplitude(const OscParams& p, const double* __restr
         s[k] = sin(arg);
         c[k] = cos(arg);
     }
-    Cd X[3][3];
+#pragma unroll
+    for (int n = 0; n < 3; n++)
+#pragma unroll
+        for (int m = 0; m < 3; m++) A[n][m] = Cd{0.0, 0.0};
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -197,23 +186,56 @@ __device__ inline void layer_amplitude(const OscParams& p, const double* __restr
                 x.re += c[k] * P[k].re - s[k] * P[k].im;
                 x.im += c[k] * P[k].im + s[k] * P[k].re;
             }
-            X[i][j] = x;
-        }
 #pragma unroll
-    for (int n = 0; n < 3; n++)
+            for (int n = 0; n < 3; n++)
 #pragma unroll
-        for (int m = 0; m < 3; m++) {
-            Cd a{0.0, 0.0};
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
+                for (int m = 0; m < 3; m++) {
                     const double* f = ax + (n * 108 + m * 36 + i * 12 + j * 4);
-                    a.re += f[0] * X[i][j].re + f[1] * X[i][j].im;
-                    a.im += f[2] * X[i][j].im + f[3] * X[i][j].re;
+                    A[n][m].re += f[0] * x.re + f[1] * x.im;
+                    A[n][m].im += f[2] * x.im + f[3] * x.re;
                 }
-            A[n][m] = a;
         }
+}
+
+// B <- A B in place, one column at a time (column j of the product needs column j of B only),
+// with the summation order of math.hpp:50-64.
+__device__ inline void mul_left(const Cd A[3][3], Cd B[3][3]) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        Cd col[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            Cd c{0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                c.re += A[i][k].re * B[k][j].re - A[i][k].im * B[k][j].im;
+                c.im += A[i][k].im * B[k][j].re + A[i][k].re * B[k][j].im;
+            }
+            col[i] = c;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++) B[i][j] = col[i];
+    }
+}
+
+// A <- A B in place, one row at a time, same summation order.
+__device__ inline void mul_right(Cd A[3][3], const Cd B[3][3]) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        Cd row[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            Cd c{0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                c.re += A[i][k].re * B[k][j].re - A[i][k].im * B[k][j].im;
+                c.im += A[i][k].im * B[k][j].re + A[i][k].re * B[k][j].im;
+            }
+            row[j] = c;
+        }
+#pragma unroll
+        for (int j = 0; j < 3; j++) A[i][j] = row[j];
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void propagate_plain_f64(GridDesc g, OscParams p, double* __restrict__ out) {
@@ -235,11 +257,12 @@ __global__ __launch_bounds__(kBlock) void propagate_plain_f64(GridDesc g, OscPar
 
     const double cz = g.cosines[ic];
     const double E = g.energies[ie];
-    const int L = g.maxlayers[ic];
+    const int L = crossed_shells(g.maxlayers, ic, g.rows);
     const double path = path_length(cz, g.h_cm);
     const double earth_path = -2.0 * cz * kREarthCm;
 
-    Cd T[3][3], fin[3][3], mantle[3][3], tmp[3][3];
+    // physics.hpp:645-677: fin = T_i fin for every layer, mantle = mantle T_i for 1 <= i < L
+    Cd T[3][3], fin[3][3], mantle[3][3];
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -248,36 +271,36 @@ __global__ __launch_bounds__(kBlock) void propagate_plain_f64(GridDesc g, OscPar
     for (int i = 0; i <= L; i++) {
         const double dist = layer_distance(g.radii, i, L, path, earth_path, cz);
         const double dens = layer_density(g.rhos, i, L);
-        layer_amplitude(p, ax, E, dens * kYe, dist / kKm2Cm, T);
+        // the AXFAC table is loop-invariant: hoisted out of the layer loop, its 324 doubles
+        // would stay live in registers across the loop and spill (1.7 KB of scratch per lane); an
+        // offset the compiler cannot see through keeps its LDS reads inside each layer step
+        int opaque_zero = 0;
+        asm volatile("" : "+s"(opaque_zero));
+        layer_amplitude(p, ax + opaque_zero, E, dens * kYe, dist / kKm2Cm, T);
         if (i == 0) {
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
                 for (int b = 0; b < 3; b++) fin[a][b] = T[a][b];
         } else {
-            mat_mul(T, fin, tmp);
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int b = 0; b < 3; b++) fin[a][b] = tmp[a][b];
-            if (i < L) {
-                mat_mul(mantle, T, tmp);
-#pragma unroll
-                for (int a = 0; a < 3; a++)
-#pragma unroll
-                    for (int b = 0; b < 3; b++) mantle[a][b] = tmp[a][b];
-            }
+            mul_left(T, fin);
+            if (i < L) mul_right(mantle, T);
         }
     }
-    mat_mul(mantle, fin, tmp);
 
+    // total = mantle fin; P(a -> b) = |total[b][a]|^2, each element formed where it is stored
     const std::size_t plane = std::size_t(g.n_c) * g.n_e;
     double* o = out + std::size_t(ic) * g.n_e + ie;
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int b = 0; b < 3; b++) {
-            const Cd v = tmp[b][a];
+            Cd v{0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                v.re += mantle[b][k].re * fin[k][a].re - mantle[b][k].im * fin[k][a].im;
+                v.im += mantle[b][k].im * fin[k][a].re + mantle[b][k].re * fin[k][a].im;
+            }
             o[(a * 3 + b) * plane] = v.re * v.re + v.im * v.im;
         }
 }
