@@ -447,34 +447,85 @@ __device__ inline void real_eigenvectors_12(const OscParams& p, double E, double
     }
 }
 
-__device__ inline void spectrum_cell_mixed(const GridDesc& g, const OscParams& p, int ne_pad, int s, int e,
-                                           float* __restrict__ Qm, double* __restrict__ Rm) {
-    double E, rho;
-    slot_inputs(g, s, e, E, rho);
-    double v[2][3], mm[3][3], mv[3][3];
-    real_eigenvectors_12(p, E, rho, v, mm, mv);
+// Real orthonormal eigenbasis O = [v_1 x v_2, v_1, v_2] (det +1) from v_1, v_2, as columns
+// o[k][component].
+__device__ inline void eigenbasis(const double v[2][3], double o[3][3]) {
+    o[0][0] = v[0][1] * v[1][2] - v[0][2] * v[1][1];
+    o[0][1] = v[0][2] * v[1][0] - v[0][0] * v[1][2];
+    o[0][2] = v[0][0] * v[1][1] - v[0][1] * v[1][0];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        o[1][c] = v[0][c];
+        o[2][c] = v[1][c];
+    }
+}
+
+// Spectrum cell (slot s, energy e).  Without kBasis one lane decomposes slot s and stores v_1,
+// v_2 and the phase rates (half = 0).  With kBasis the cell has two adjacent lanes of one
+// wavefront (half = the lane's parity): lane 0 does the above; lane 1 decomposes slot s - 1
+// (O_0 = I for s = 1) and, with lane 0's eigenbasis, stores the basis change M_s = O_s^T O_{s-1}
+// (row r: (o_r . o'_c)_c).  Both lanes reach the exchange; each does one decomposition, so the
+// cell costs the latency of one.  M_0 is not used.
+template <bool kBasis>
+__device__ inline void spectrum_cell_mixed(const GridDesc& g, const OscParams& p, int ne_pad, int s, int e, int half,
+                                           float* __restrict__ Qm, float* __restrict__ Mm, double* __restrict__ Rm) {
     const std::size_t st = ne_pad;
-    float4* dst = reinterpret_cast<float4*>(Qm);
-    dst[(std::size_t(s) * kMixedQGroups + 0) * st + e] =
-        make_float4(float(v[0][0]), float(v[0][1]), float(v[0][2]), float(v[1][0]));
-    dst[(std::size_t(s) * kMixedQGroups + 1) * st + e] = make_float4(float(v[1][1]), float(v[1][2]), 0.0f, 0.0f);
-    const double half_turn = 1.0 / (E * kPi);         // radians -> half turns
-    Rm[(std::size_t(s) * st + e) * 2 + 0] = -kLoE * mm[1][0] * half_turn;
-    Rm[(std::size_t(s) * st + e) * 2 + 1] = -kLoE * mm[2][0] * half_turn;
+    double o[3][3];
+    if (half == 0 || s >= 2) {
+        double E, rho;
+        slot_inputs(g, s - half, e, E, rho);
+        double v[2][3], mm[3][3], mv[3][3];
+        real_eigenvectors_12(p, E, rho, v, mm, mv);
+        eigenbasis(v, o);
+        if (half == 0) {
+            float4* dst = reinterpret_cast<float4*>(Qm);
+            dst[(std::size_t(s) * kMixedQGroups + 0) * st + e] =
+                make_float4(float(v[0][0]), float(v[0][1]), float(v[0][2]), float(v[1][0]));
+            dst[(std::size_t(s) * kMixedQGroups + 1) * st + e] =
+                make_float4(float(v[1][1]), float(v[1][2]), 0.0f, 0.0f);
+            const double half_turn = 1.0 / (E * kPi);         // radians -> half turns
+            Rm[(std::size_t(s) * st + e) * 2 + 0] = -kLoE * mm[1][0] * half_turn;
+            Rm[(std::size_t(s) * st + e) * 2 + 1] = -kLoE * mm[2][0] * half_turn;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) o[r][c] = r == c ? 1.0 : 0.0;
+    }
+    if constexpr (!kBasis) return;
+    double cur[3][3];                 // lane 0's eigenbasis O_s, on both lanes
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) cur[r][c] = __shfl_xor(o[r][c], 1);
+    if (half == 1) {
+        float4* mdst = reinterpret_cast<float4*>(Mm);
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            float m[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                m[c] = float(cur[r][0] * o[c][0] + cur[r][1] * o[c][1] + cur[r][2] * o[c][2]);
+            mdst[(std::size_t(s) * kMixedMGroups + r) * st + e] = make_float4(m[0], m[1], m[2], 0.0f);
+        }
+    }
 }
 
 // Set-up of one mixed-precision grid in one launch: cells [0, G) are layer lengths
 // (cosine, layer), [G, G + S) spectrum cells (slot, energy).
 __global__ __launch_bounds__(kBlock) void grid_setup_mixed(GridDesc g, OscParams p, int ne_pad, double* __restrict__ geom,
-                                                           float* __restrict__ Qm, double* __restrict__ Rm) {
+                                                           float* __restrict__ Qm, float* __restrict__ Mm,
+                                                           double* __restrict__ Rm) {
     const int slots = g.rows + 1;
     const int G = g.n_c * slots, S = slots * ne_pad;
+    const int G2 = (G + 1) & ~1;       // spectrum lanes start even: a cell's two lanes share a wavefront
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < G) {
         geom[t] = geometry_cell(g, t / slots, t % slots);
-    } else if (t < G + S) {
-        const int u = t - G;
-        spectrum_cell_mixed(g, p, ne_pad, u / ne_pad, u % ne_pad, Qm, Rm);
+    } else if (t >= G2 && t < G2 + 2 * S) {
+        const int cell = (t - G2) >> 1;
+        spectrum_cell_mixed<true>(g, p, ne_pad, cell / ne_pad, cell % ne_pad, (t - G2) & 1, Qm, Mm, Rm);
     }
 }
 
@@ -489,8 +540,8 @@ __device__ inline int find_set(const long long* __restrict__ off, int n, long lo
 }
 
 // Batched set-up of many parameter sets in one launch: work cells [0, G) are layer lengths
-// (set, cosine, layer), [G, G + S) spectrum cells (set, slot, energy).  Also resets the pair
-// kernel's work queue.
+// (set, cosine, layer), [G, G + S) spectrum cells (set, slot, energy) of the rank-1 tables (the
+// batched kernel needs no basis changes).  Also resets the pair kernel's work queue.
 __global__ __launch_bounds__(kBlock) void batch_setup(BatchDesc b) {
     const long long G = b.geom_prefix[b.n_sets], S = b.spec_prefix[b.n_sets];
     if (blockIdx.x == 0 && threadIdx.x < kXcds) b.next_item[threadIdx.x * kQueueStride] = 0;   // pair kernel follows
@@ -509,9 +560,9 @@ __global__ __launch_bounds__(kBlock) void batch_setup(BatchDesc b) {
             const BatchSet& set = b.sets[k];
             const GridDesc g = batch_grid(b, set);
             const long long c = u - b.spec_prefix[k];
-            spectrum_cell_mixed(g, set.p, set.ne_pad, int(c / set.ne_pad), int(c % set.ne_pad),
-                                reinterpret_cast<float*>(b.Q + set.tab_off * kMixedQGroups),
-                                reinterpret_cast<double*>(b.R + set.tab_off));
+            spectrum_cell_mixed<false>(g, set.p, set.ne_pad, int(c / set.ne_pad), int(c % set.ne_pad), 0,
+                                       reinterpret_cast<float*>(b.Q + set.tab_off * kMixedQGroups), nullptr,
+                                       reinterpret_cast<double*>(b.R + set.tab_off));
         }
     }
 }
@@ -533,12 +584,13 @@ hipError_t launch_geometry(const GridDesc& g, double* geom, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_setup_mixed(const GridDesc& g, const OscParams& p, int ne_pad, double* geom, float* Qm, double* Rm,
-                              hipStream_t s) {
+hipError_t launch_setup_mixed(const GridDesc& g, const OscParams& p, int ne_pad, double* geom, float* Qm, float* Mm,
+                              double* Rm, hipStream_t s) {
     if (g.n_c == 0 || g.n_e == 0) return hipSuccess;
-    const unsigned n = unsigned(g.n_c) * (g.rows + 1) + unsigned(g.rows + 1) * ne_pad;
+    const unsigned G = unsigned(g.n_c) * (g.rows + 1);
+    const unsigned n = ((G + 1) & ~1u) + 2u * unsigned(g.rows + 1) * ne_pad;
     hipLaunchKernelGGL(grid_setup_mixed, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g, p, ne_pad, geom,
-                       Qm, Rm);
+                       Qm, Mm, Rm);
     return hipGetLastError();
 }
 

@@ -79,18 +79,25 @@ hipError_t launch_geometry(const GridDesc& g, double* geom, hipStream_t s);
 //         -2.534 lambda_k (phase R*L/E).
 //  mixed: Qm float4 groups [(s*2 + g)*ne_pad + e] holding the real unit eigenvectors v_1, v_2
 //         of the layer Hamiltonian in the real basis (v_1, then v_2, then 2 zero),
+//         Mm float4 groups [(s*3 + r)*ne_pad + e] holding row r of the basis change
+//         M_s = O_s^T O_{s-1} between the eigenbases O = [v_1 x v_2, v_1, v_2] of slots s and
+//         s-1 (O_0 taken as I; M_0 unused), and
 //         Rm[(s*ne_pad + e)*2 + k-1] = -2.534 (lambda_k - lambda_0) / E in half turns per km.
 constexpr int kMixedQGroups = 2;
 constexpr int kMixedQFloats = 4 * kMixedQGroups;
+constexpr int kMixedMGroups = 3;
+constexpr int kMixedMFloats = 4 * kMixedMGroups;
 constexpr int kMixedRDoubles = 2;
-// Mixed set-up in one launch: layer lengths (geom) and the mixed spectrum table.
-hipError_t launch_setup_mixed(const GridDesc& g, const OscParams& p, int ne_pad, double* geom, float* Qm, double* Rm,
-                              hipStream_t s);
+// bytes of the mixed tables per (slot, energy) cell
+constexpr int kMixedCellBytes = kMixedQFloats * 4 + kMixedMFloats * 4 + kMixedRDoubles * 8;
+// Mixed set-up in one launch: layer lengths (geom) and the mixed spectrum tables.
+hipError_t launch_setup_mixed(const GridDesc& g, const OscParams& p, int ne_pad, double* geom, float* Qm, float* Mm,
+                              double* Rm, hipStream_t s);
 hipError_t launch_spectrum_f64(const GridDesc& g, const OscParams& p, int ne_pad, double* Q, double* R, hipStream_t s);
 
 // Table-driven per-pair kernels.
 // wmu: row 1 of the real basis's W in float (OscParams::Wmu)
-hipError_t launch_pairs_mixed(const GridDesc& g, int ne_pad, const double* geom, const float* Q,
+hipError_t launch_pairs_mixed(const GridDesc& g, int ne_pad, const double* geom, const float* Q, const float* M,
                               const double* R, float4 wmu, float* out, hipStream_t s);
 hipError_t launch_pairs_f64(const GridDesc& g, int ne_pad, const double* geom, const double* Q,
                             const double* R, double* out, hipStream_t s);
@@ -106,8 +113,8 @@ hipError_t launch_sum_f64(const double* in, long long n, double* scratch, hipStr
 // its pairs and nine channels, into partials[likelihood_blocks(g)]; nothing else is stored.
 unsigned likelihood_blocks(const GridDesc& g);
 hipError_t launch_pairs_mixed_sqdiff(const GridDesc& g, int ne_pad, const double* geom, const float* Q,
-                                     const double* R, float4 wmu, const float* data, double* partials,
-                                     hipStream_t s);
+                                     const float* M, const double* R, float4 wmu, const float* data,
+                                     double* partials, hipStream_t s);
 // sum_i (a_i - b_i)^2 in FP64 for device floats; result in partials[reduce_partials_needed()].
 hipError_t launch_sqdiff_sum(const float* a, const float* b, std::size_t n, double* partials, hipStream_t s);
 
@@ -150,7 +157,7 @@ struct BatchDesc {
     const double* radii;
     const double* rhos;
     double* geom;
-    float4* Q;                      // kMixedQGroups float4 per spectrum cell
+    float4* Q;                      // kMixedQGroups float4 per spectrum cell (rank-1 tables: no basis changes)
     double2* R;
     unsigned long long* next_item;  // kXcds work-queue counters of the pair kernel, kQueueStride apart
                                     // (zeroed by the set-up kernel)

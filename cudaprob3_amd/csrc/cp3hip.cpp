@@ -38,6 +38,7 @@ struct cp3h_propagator {
     double* d_rhos = nullptr;
     double* d_geom = nullptr;
     void* d_Q = nullptr;
+    float* d_M = nullptr;              // mixed only: basis changes between shells
     double* d_R = nullptr;
     void* d_out = nullptr;
     int out_bytes = 8;
@@ -145,6 +146,7 @@ int cp3h_destroy(cp3h_propagator* h) {
     free_dev(h->d_rhos);
     free_dev(h->d_geom);
     free_dev(h->d_Q);
+    free_dev(h->d_M);
     free_dev(h->d_R);
     free_dev(h->d_out);
     free_dev(h->d_partials);
@@ -225,6 +227,7 @@ int cp3h_set_density_f64(cp3h_propagator* h, const double* radii_desc, const dou
         free_dev(h->d_rhos);
         free_dev(h->d_geom);
         free_dev(h->d_Q);
+        free_dev(h->d_M);
         free_dev(h->d_R);
         h->rows = 0;
         h->has_density = false;
@@ -236,6 +239,7 @@ int cp3h_set_density_f64(cp3h_propagator* h, const double* radii_desc, const dou
             const std::size_t cells = slots * h->ne_pad;
             const bool mixed = h->precision == CP3H_MIXED;
             CP3H_HIP(hipMalloc(&h->d_Q, cells * (mixed ? cp3::kMixedQFloats * sizeof(float) : 27 * sizeof(double))));
+            if (mixed) CP3H_HIP(hipMalloc(&h->d_M, cells * cp3::kMixedMFloats * sizeof(float)));
             CP3H_HIP(hipMalloc(&h->d_R, cells * (mixed ? cp3::kMixedRDoubles : 3) * sizeof(double)));
         }
         h->rows = n_rows;
@@ -304,8 +308,8 @@ int ready_to_compute(cp3h_propagator* h, int antineutrino, double production_hei
 int enqueue_setup(cp3h_propagator* h, const cp3::GridDesc& g) {
     if (h->n_c == 0 || h->n_e == 0 || h->precision == CP3H_FP64_PLAIN) return CP3H_OK;
     if (h->precision == CP3H_MIXED) {
-        CP3H_HIP(cp3::launch_setup_mixed(g, h->params, h->ne_pad, h->d_geom, static_cast<float*>(h->d_Q), h->d_R,
-                                         h->stream));
+        CP3H_HIP(cp3::launch_setup_mixed(g, h->params, h->ne_pad, h->d_geom, static_cast<float*>(h->d_Q), h->d_M,
+                                         h->d_R, h->stream));
     } else {
         CP3H_HIP(cp3::launch_geometry(g, h->d_geom, h->stream));
         CP3H_HIP(cp3::launch_spectrum_f64(g, h->params, h->ne_pad, static_cast<double*>(h->d_Q), h->d_R, h->stream));
@@ -321,11 +325,11 @@ int enqueue_pairs(cp3h_propagator* h, const cp3::GridDesc& g, const float* data)
         CP3H_HIP(cp3::launch_plain_f64(g, h->params, static_cast<double*>(h->d_out), h->stream));
     } else if (h->precision == CP3H_MIXED) {
         if (data)
-            CP3H_HIP(cp3::launch_pairs_mixed_sqdiff(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_R,
-                                                    wmu_f32(h->params), data, h->d_like, h->stream));
+            CP3H_HIP(cp3::launch_pairs_mixed_sqdiff(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_M,
+                                                    h->d_R, wmu_f32(h->params), data, h->d_like, h->stream));
         else
-            CP3H_HIP(cp3::launch_pairs_mixed(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_R,
-                                             wmu_f32(h->params), static_cast<float*>(h->d_out), h->stream));
+            CP3H_HIP(cp3::launch_pairs_mixed(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_M,
+                                             h->d_R, wmu_f32(h->params), static_cast<float*>(h->d_out), h->stream));
     } else {
         CP3H_HIP(cp3::launch_pairs_f64(g, h->ne_pad, h->d_geom, static_cast<const double*>(h->d_Q), h->d_R,
                                        static_cast<double*>(h->d_out), h->stream));

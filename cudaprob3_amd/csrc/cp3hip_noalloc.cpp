@@ -25,7 +25,7 @@
 #include <string>
 
 namespace {
-constexpr std::size_t kScratchBytes = std::size_t(64) << 20;   // 64 MiB per GPU
+constexpr std::size_t kScratchBytes = std::size_t(128) << 20;   // 128 MiB per GPU
 constexpr int kMaxDevices = 64;
 }  // namespace
 
@@ -80,13 +80,14 @@ extern "C" int cp3h_propagate_mixed_noalloc(int device, const double* cosines_de
     const std::size_t ne_pad = cp3::round_up(unsigned(n_energies > 0 ? n_energies : 1), cp3::kWave);
     const std::size_t geom_bytes = align256(sizeof(double) * std::size_t(n_cosines) * slots);
     const std::size_t q_bytes = align256(sizeof(float) * cp3::kMixedQFloats * slots * ne_pad);
+    const std::size_t m_bytes = align256(sizeof(float) * cp3::kMixedMFloats * slots * ne_pad);
     const std::size_t r_bytes = align256(sizeof(double) * cp3::kMixedRDoubles * slots * ne_pad);
-    const std::size_t need = geom_bytes + q_bytes + r_bytes;
+    const std::size_t need = geom_bytes + q_bytes + m_bytes + r_bytes;
     if (need > kScratchBytes)
         return fail(CP3H_ERR_ARGUMENT, "grid too large for the allocation-free scratch: needs " +
                                            std::to_string(need) + " bytes, the limit is " +
                                            std::to_string(kScratchBytes) +
-                                           " (8 n_cosines (n_rows+1) + 48 (n_rows+1) n_energies rounded up to 64)");
+                                           " (8 n_cosines (n_rows+1) + 96 (n_rows+1) n_energies rounded up to 64)");
 
     int n_dev = 0;
     if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev == 0) return fail(CP3H_ERR_DEVICE, "No GPU found");
@@ -108,11 +109,12 @@ extern "C" int cp3h_propagate_mixed_noalloc(int device, const double* cosines_de
 
     double* geom = reinterpret_cast<double*>(base);
     float* Q = reinterpret_cast<float*>(base + geom_bytes);
-    double* R = reinterpret_cast<double*>(base + geom_bytes + q_bytes);
+    float* M = reinterpret_cast<float*>(base + geom_bytes + q_bytes);
+    double* R = reinterpret_cast<double*>(base + geom_bytes + q_bytes + m_bytes);
     const cp3::GridDesc g{cosines_device, energies_device, maxlayers_device, radii_desc_device, rhos_desc_device,
                           n_cosines, n_energies, n_rows, production_height_cm};
     hipStream_t s = static_cast<hipStream_t>(stream);
-    CP3H_HIP(cp3::launch_setup_mixed(g, p, int(ne_pad), geom, Q, R, s));
-    CP3H_HIP(cp3::launch_pairs_mixed(g, int(ne_pad), geom, Q, R, wmu, out_device, s));
+    CP3H_HIP(cp3::launch_setup_mixed(g, p, int(ne_pad), geom, Q, M, R, s));
+    CP3H_HIP(cp3::launch_pairs_mixed(g, int(ne_pad), geom, Q, M, R, wmu, out_device, s));
     return CP3H_OK;
 }

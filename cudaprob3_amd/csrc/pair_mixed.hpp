@@ -4,8 +4,8 @@
 // propagate_pairs.hip.
 //   L    : shells crossed (maxlayers of the rows)
 //   gl[k]: layer lengths of row k in km, gl[k][0..L]
-//   Q, R : the spectrum table (float4 groups / double2 phase rates in half turns per km),
-//          row stride st
+//   Q, M, R : the spectrum tables (float4 groups of v_1, v_2 / of the basis changes M_s /
+//          double2 phase rates in half turns per km), row stride st
 //   ie   : energy index (< ne_pad)
 // Writes p[k][in][out] = P(in -> out) of row k.
 #pragma once
@@ -66,6 +66,28 @@ __device__ inline void phase_factor_m1(double t, float& cm1, float& sn) {
     const unsigned lo = unsigned(__builtin_bit_cast(unsigned long long, q + 6755399441055744.0));
     const float sign = __builtin_bit_cast(float, (lo << 31) | 0x3f800000u);
     cm1 = __builtin_fmaf(sign, c0m1, sign - 1.0f);
+    sn = sign * s0;
+}
+
+// e^{i phi} for a phase in half turns: the reduction and polynomials of phase_factor_m1, with
+// cos phi = sign (1 + (cos z - 1)) and sin phi = sign sin z, each with one rounding.
+__device__ inline void phase_factor(double t, float& c, float& sn) {
+#pragma clang fp contract(off)
+    const double q = rint(t);
+    const float z = float((t - q) * 3.141592653589793);
+    const float z2 = z * z;
+    float ps = __builtin_fmaf(z2, 2.600053676e-6f, -1.980661473e-4f);
+    ps = __builtin_fmaf(z2, ps, 8.333017118e-3f);
+    ps = __builtin_fmaf(z2, ps, -1.666665673e-1f);
+    const float s0 = __builtin_fmaf(z * z2, ps, z);
+    float pc = __builtin_fmaf(z2, -2.607709746e-7f, 2.476188638e-5f);
+    pc = __builtin_fmaf(z2, pc, -1.388840377e-3f);
+    pc = __builtin_fmaf(z2, pc, 4.166664183e-2f);
+    pc = __builtin_fmaf(z2, pc, -0.5f);
+    const float c0m1 = z2 * pc;
+    const unsigned lo = unsigned(__builtin_bit_cast(unsigned long long, q + 6755399441055744.0));
+    const float sign = __builtin_bit_cast(float, (lo << 31) | 0x3f800000u);
+    c = __builtin_fmaf(sign, c0m1, sign);
     sn = sign * s0;
 }
 
@@ -148,40 +170,192 @@ __device__ inline __attribute__((always_inline)) void init_layer(const LayerTabl
         }
 }
 
+// One shell's basis change M_s = O_s^T O_{s-1} (rows) and phase rates at this lane's energy.
+struct ShellBasis {
+    float m[3][3];
+    double2 r;
+};
+
+__device__ inline __attribute__((always_inline)) void load_basis(const float4* __restrict__ M,
+                                                                const double2* __restrict__ R, std::size_t st,
+                                                                unsigned off, int i, ShellBasis& b) {
+    const char* m0 = reinterpret_cast<const char*>(M + std::size_t(i) * kMixedMGroups * st);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const float4 a = *reinterpret_cast<const float4*>(m0 + r * st * sizeof(float4) + off);
+        b.m[r][0] = a.x; b.m[r][1] = a.y; b.m[r][2] = a.z;
+    }
+    b.r = *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(R + std::size_t(i) * st) + off);
+}
+
+// d_k = e^{i dphi_k} (k = 1, 2) of a layer of length dist km
+struct LayerDiag {
+    float c[2], s[2];
+};
+
+__device__ inline __attribute__((always_inline)) LayerDiag layer_diag(double2 r, double dist) {
+    LayerDiag d;
+    phase_factor(r.x * dist, d.c[0], d.s[0]);
+    phase_factor(r.y * dist, d.c[1], d.s[1]);
+    return d;
+}
+
+// Z <- D M Z for a real 3x3 M and D = diag(1, d_1, d_2): 3 x kCols complex Z.
+template <int kCols>
+__device__ inline __attribute__((always_inline)) void basis_step(const float m[3][3], const LayerDiag* d,
+                                                                 Cx<float> Z[3][kCols]) {
+#pragma clang fp contract(off)
+    Cx<float> N[3][kCols];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int j = 0; j < kCols; j++) {
+            float re = m[a][0] * Z[0][j].re, im = m[a][0] * Z[0][j].im;
+            re = __builtin_fmaf(m[a][1], Z[1][j].re, re);
+            im = __builtin_fmaf(m[a][1], Z[1][j].im, im);
+            re = __builtin_fmaf(m[a][2], Z[2][j].re, re);
+            im = __builtin_fmaf(m[a][2], Z[2][j].im, im);
+            N[a][j] = Cx<float>{re, im};
+        }
+#pragma unroll
+    for (int j = 0; j < kCols; j++) Z[0][j] = N[0][j];
+#pragma unroll
+    for (int a = 1; a < 3; a++)
+#pragma unroll
+        for (int j = 0; j < kCols; j++) {
+            if (d) {
+                const float c = d->c[a - 1], sn = d->s[a - 1];
+                float re = c * N[a][j].re, im = c * N[a][j].im;
+                re = __builtin_fmaf(-sn, N[a][j].im, re);
+                im = __builtin_fmaf(sn, N[a][j].re, im);
+                Z[a][j] = Cx<float>{re, im};
+            } else {
+                Z[a][j] = N[a][j];
+            }
+        }
+}
+
+// Atmosphere: G = S_0 W^dagger[:, (e, mu)] = W^dagger[:, (e, mu)] + sum_k v_k w_k z_k^T,
+// z_k = W[(e, mu), :] v_k = (v_k0, W_mu1 v_k1 + W_mu2 v_k2) conjugated; t: slot 0's table entries.
+__device__ inline __attribute__((always_inline)) void atmosphere_columns(const LayerTable& t, double dist, float4 wmu,
+                                                                         Cx<float> g[3][2]) {
+#pragma clang fp contract(off)
+    const LayerPhases w = layer_phases(t, dist);
+    g[0][0] = Cx<float>{1.0f, 0.0f}; g[1][0] = Cx<float>{0.0f, 0.0f}; g[2][0] = Cx<float>{0.0f, 0.0f};
+    g[0][1] = Cx<float>{0.0f, 0.0f}; g[1][1] = Cx<float>{wmu.x, -wmu.y}; g[2][1] = Cx<float>{wmu.z, -wmu.w};
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++) {
+        const float* v = t.v[kk];
+        const float zr = __builtin_fmaf(wmu.z, v[2], wmu.x * v[1]);
+        const float zi = -__builtin_fmaf(wmu.w, v[2], wmu.y * v[1]);
+        const float er = w.re[kk] * v[0], ei = w.im[kk] * v[0];
+        float mr = w.re[kk] * zr, mi = w.re[kk] * zi;
+        mr = __builtin_fmaf(-w.im[kk], zi, mr);
+        mi = __builtin_fmaf(w.im[kk], zr, mi);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            g[a][0].re = __builtin_fmaf(v[a], er, g[a][0].re);
+            g[a][0].im = __builtin_fmaf(v[a], ei, g[a][0].im);
+            g[a][1].re = __builtin_fmaf(v[a], mr, g[a][1].re);
+            g[a][1].im = __builtin_fmaf(v[a], mi, g[a][1].im);
+        }
+    }
+}
+
+// p[in][out] = |T[out][in]|^2 from the 2 x 2 block T (out, in in (e, mu)); the third row and
+// column from unitarity
+__device__ inline __attribute__((always_inline)) void block_probabilities(const Cx<float> T[2][2], float (&pk)[3][3]) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int in = 0; in < 2; in++)
+#pragma unroll
+        for (int o = 0; o < 2; o++)
+            pk[in][o] = __builtin_fmaf(T[o][in].re, T[o][in].re, T[o][in].im * T[o][in].im);
+    pk[0][2] = clamp01(1.0f - pk[0][0] - pk[0][1]);
+    pk[1][2] = clamp01(1.0f - pk[1][0] - pk[1][1]);
+    pk[2][0] = clamp01(1.0f - pk[0][0] - pk[1][0]);
+    pk[2][1] = clamp01(1.0f - pk[0][1] - pk[1][1]);
+    pk[2][2] = clamp01(1.0f - pk[2][0] - pk[2][1]);
+}
+
 // The path's layers (physics.hpp:645-677): layer 0 (atmosphere), shells 1..L-1 on the way in,
 // the innermost shell L, shells L-1..1 on the way out; total T = T_1..T_{L-1} T_L T_{L-1}..T_1 T_0.
-// In the real basis every layer amplitude is W S_i W^dagger with S_i complex symmetric, so with
-// Y = S_{L-1} ... S_1 (one running product over the shells of the way in) the way out is
-// S_1 ... S_{L-1} = Y^T, and
-//   T = W Y^T S_L Y S_0 W^dagger.
-// Only the flavour rows and columns e, mu of T are needed (unitarity gives the rest): with
-// G = S_0 W^dagger[:, e mu] (3 x 2) and the rows e, mu of W Y^T, the 2 x 2 block is
-// (W Y^T)[e mu, :] S_L (Y G).  Per shell of the way in one 3x3 update (two rank-1 terms); the
-// atmosphere, the innermost shell and the rotation are applied once per pair.  L is
-// wave-uniform, so every branch here is a scalar one.
+// In the real basis every layer amplitude is W S_i W^dagger with S_i = O_i D_i O_i^T complex
+// symmetric (O_i the real eigenbasis, D_i = diag(1, e^{i dphi_1}, e^{i dphi_2}) with the global
+// phase dropped: it cancels in |T|^2), so with Y = S_{L-1} ... S_1 (the way in) the way out
+// S_1 ... S_{L-1} is Y^T, and
+//   T = W Y^T S_L Y G,   G = S_0 W^dagger[:, (e, mu)].
+// Only the flavour rows and columns e, mu of T are needed (unitarity gives the rest).  Two forms
+// of the running product for L >= 2:
+//  * eigenbasis (kEigen, single-grid kernels): Y = O_{L-1} Z with Z = D_{L-1} M_{L-1} ... D_1 M_1,
+//    M_s = O_s^T O_{s-1} (O_0 = I) from the table: one shell costs a real 3x3 times complex 3x3
+//    product and two complex row scalings.  With Z' = M_L Z = O_L^T Y,
+//      T[x][y] = sum_c (Z' a_x)_c D_L,c (Z' g_y)_c,
+//    a_e = e_0, a_mu = row mu of W (as a column), g_y = G[:, y];
+//  * rank-1 (batched kernel, whose tables then need no basis changes): Y += v_k (w_k v_k^T Y)
+//    for k = 1, 2 per shell (w_k = e^{i dphi_k} - 1, sum_k v_k v_k^T = I), then
+//    T = (W Y^T)[e mu, :] S_L (Y G).
+// L <= 1: T = W S_1 G (S_1 as two rank-1 updates of G) or W G.
 //
 // kRows cosine rows with the same L are propagated together at this lane's energy: they cross
 // the same shells, so each shell's table entries are loaded once for all of them (only the
 // layer lengths, hence the phases, differ), which halves the table reads per pair for kRows = 2.
-// wmu = row mu of W (OscParams::Wmu); row e of W is e_0^T.
-template <int kRows>
+// wmu = row mu of W (OscParams::Wmu); row e of W is e_0^T.  L is wave-uniform, so every branch
+// here is a scalar one.
+template <int kRows, bool kEigen>
 __device__ inline void propagate_rows_mixed(int L, const double* const* gl, const float4* __restrict__ Q,
-                                            const double2* __restrict__ R, std::size_t st, int ie, float4 wmu,
-                                            float (*p)[3][3]) {
+                                            const float4* __restrict__ M, const double2* __restrict__ R,
+                                            std::size_t st, int ie, float4 wmu, float (*p)[3][3]) {
 #pragma clang fp contract(off)
     static_assert(sizeof(float4) == 16 && sizeof(double2) == 16, "table entries are 16 bytes");
     const unsigned off = unsigned(ie) * 16u;
+    const Cx<float> w1{wmu.x, wmu.y}, w2{wmu.z, wmu.w};
     LayerTable t;
-    Cx<float> Y[kRows][3][3];   // S_{i} ... S_1
-    Cx<float> G[kRows][3][2];   // columns e, mu of ... S_0 W^dagger
-    if (L >= 2) {
-        load_table(Q, R, st, off, 1, t);
+    Cx<float> G[kRows][3][2];   // columns e, mu of S_0 W^dagger
+    if (L < 2) {
+        load_table(Q, R, st, off, 0, t);
 #pragma unroll
-        for (int k = 0; k < kRows; k++) init_layer(t, layer_phases(t, gl[k][1]), Y[k]);
+        for (int k = 0; k < kRows; k++) atmosphere_columns(t, gl[k][0], wmu, G[k]);
+        if (L == 1) {
+            load_table(Q, R, st, off, 1, t);
+#pragma unroll
+            for (int k = 0; k < kRows; k++) apply_layer<2>(t, layer_phases(t, gl[k][1]), G[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kRows; k++) {
+            Cx<float> T[2][2];
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                T[0][b] = G[k][0][b];
+                Cx<float> m = cmul(w1, G[k][1][b]);
+                cmac(m, w2, G[k][2][b]);
+                T[1][b] = m;
+            }
+            block_probabilities(T, p[k]);
+        }
+        return;
+    }
+    if constexpr (kEigen) {
+        ShellBasis b;
+        Cx<float> Z[kRows][3][3];
+        load_basis(M, R, st, off, 1, b);
+#pragma unroll
+        for (int k = 0; k < kRows; k++) {           // Z = D_1 M_1
+            const LayerDiag d = layer_diag(b.r, gl[k][1]);
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                Z[k][0][j] = Cx<float>{b.m[0][j], 0.0f};
+#pragma unroll
+                for (int a = 1; a < 3; a++) Z[k][a][j] = Cx<float>{d.c[a - 1] * b.m[a][j], d.s[a - 1] * b.m[a][j]};
+            }
+        }
         auto shell = [&](int i) {
-            load_table(Q, R, st, off, i, t);
+            load_basis(M, R, st, off, i, b);
 #pragma unroll
-            for (int k = 0; k < kRows; k++) apply_layer<3>(t, layer_phases(t, gl[k][i]), Y[k]);
+            for (int k = 0; k < kRows; k++) {
+                const LayerDiag d = layer_diag(b.r, gl[k][i]);
+                basis_step<3>(b.m, &d, Z[k]);
+            }
         };
         // one row: unrolled twice for independent work across shells; two rows carry enough
         // independent work as they are
@@ -192,38 +366,70 @@ __device__ inline void propagate_rows_mixed(int L, const double* const* gl, cons
 #pragma unroll 1
             for (int i = 2; i < L; i++) shell(i);
         }
-    }
-    // atmosphere: G = S_0 W^dagger[:, (e, mu)] = W^dagger[:, (e, mu)] + sum_k v_k w_k z_k^T,
-    // z_k = W[(e, mu), :] v_k = (v_k0, W_mu1 v_k1 + W_mu2 v_k2) conjugated
-    load_table(Q, R, st, off, 0, t);
+        load_basis(M, R, st, off, L, b);
+        LayerDiag dL[kRows];
 #pragma unroll
-    for (int k = 0; k < kRows; k++) {
-        const LayerPhases w = layer_phases(t, gl[k][0]);
-        Cx<float>(&g)[3][2] = G[k];
-        g[0][0] = Cx<float>{1.0f, 0.0f}; g[1][0] = Cx<float>{0.0f, 0.0f}; g[2][0] = Cx<float>{0.0f, 0.0f};
-        g[0][1] = Cx<float>{0.0f, 0.0f}; g[1][1] = Cx<float>{wmu.x, -wmu.y}; g[2][1] = Cx<float>{wmu.z, -wmu.w};
-#pragma unroll
-        for (int kk = 0; kk < 2; kk++) {
-            const float* v = t.v[kk];
-            const float zr = __builtin_fmaf(wmu.z, v[2], wmu.x * v[1]);
-            const float zi = -__builtin_fmaf(wmu.w, v[2], wmu.y * v[1]);
-            const float er = w.re[kk] * v[0], ei = w.im[kk] * v[0];
-            float mr = w.re[kk] * zr, mi = w.re[kk] * zi;
-            mr = __builtin_fmaf(-w.im[kk], zi, mr);
-            mi = __builtin_fmaf(w.im[kk], zr, mi);
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                g[a][0].re = __builtin_fmaf(v[a], er, g[a][0].re);
-                g[a][0].im = __builtin_fmaf(v[a], ei, g[a][0].im);
-                g[a][1].re = __builtin_fmaf(v[a], mr, g[a][1].re);
-                g[a][1].im = __builtin_fmaf(v[a], mi, g[a][1].im);
-            }
+        for (int k = 0; k < kRows; k++) {
+            basis_step<3>(b.m, nullptr, Z[k]);      // Z' = M_L Z
+            dL[k] = layer_diag(b.r, gl[k][L]);
         }
-    }
-    if (L >= 2) {
+        load_table(Q, R, st, off, 0, t);
 #pragma unroll
-        for (int k = 0; k < kRows; k++) {           // G <- Y G
-            Cx<float> N[3][2];
+        for (int k = 0; k < kRows; k++) {
+            atmosphere_columns(t, gl[k][0], wmu, G[k]);
+            // h_x = D_L Z' a_x for x = e, mu, and u_y = Z' g_y
+            Cx<float> h[2][3], u[3][2];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                h[0][c] = Z[k][c][0];
+                h[1][c] = cmul(w1, Z[k][c][1]);
+                cmac(h[1][c], w2, Z[k][c][2]);
+#pragma unroll
+                for (int y = 0; y < 2; y++) {
+                    Cx<float> acc = cmul(Z[k][c][0], G[k][0][y]);
+                    cmac(acc, Z[k][c][1], G[k][1][y]);
+                    cmac(acc, Z[k][c][2], G[k][2][y]);
+                    u[c][y] = acc;
+                }
+            }
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+#pragma unroll
+                for (int c = 1; c < 3; c++) h[x][c] = cmul(Cx<float>{dL[k].c[c - 1], dL[k].s[c - 1]}, h[x][c]);
+            Cx<float> T[2][2];
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+#pragma unroll
+                for (int y = 0; y < 2; y++) {
+                    Cx<float> acc = cmul(h[x][0], u[0][y]);
+                    cmac(acc, h[x][1], u[1][y]);
+                    cmac(acc, h[x][2], u[2][y]);
+                    T[x][y] = acc;
+                }
+            block_probabilities(T, p[k]);
+        }
+    } else {
+        Cx<float> Y[kRows][3][3];   // S_{i} ... S_1
+        load_table(Q, R, st, off, 1, t);
+#pragma unroll
+        for (int k = 0; k < kRows; k++) init_layer(t, layer_phases(t, gl[k][1]), Y[k]);
+        auto shell = [&](int i) {
+            load_table(Q, R, st, off, i, t);
+#pragma unroll
+            for (int k = 0; k < kRows; k++) apply_layer<3>(t, layer_phases(t, gl[k][i]), Y[k]);
+        };
+        if (kRows == 1) {
+#pragma unroll 2
+            for (int i = 2; i < L; i++) shell(i);
+        } else {
+#pragma unroll 1
+            for (int i = 2; i < L; i++) shell(i);
+        }
+        load_table(Q, R, st, off, 0, t);
+#pragma unroll
+        for (int k = 0; k < kRows; k++) {
+            atmosphere_columns(t, gl[k][0], wmu, G[k]);
+            Cx<float> N[3][2];                      // G <- Y G
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -238,19 +444,13 @@ __device__ inline void propagate_rows_mixed(int L, const double* const* gl, cons
 #pragma unroll
                 for (int b = 0; b < 2; b++) G[k][a][b] = N[a][b];
         }
-    }
-    if (L >= 1) {
         load_table(Q, R, st, off, L, t);
 #pragma unroll
         for (int k = 0; k < kRows; k++) apply_layer<2>(t, layer_phases(t, gl[k][L]), G[k]);
-    }
-    const Cx<float> w1{wmu.x, wmu.y}, w2{wmu.z, wmu.w};
 #pragma unroll
-    for (int k = 0; k < kRows; k++) {
-        // T[out][in] for out, in in (e, mu): rows e, mu of W Y^T (W for L < 2) times G
-        Cx<float> T[2][2];
-        if (L >= 2) {
-            Cx<float> He[3], Hm[3];
+        for (int k = 0; k < kRows; k++) {
+            // T[out][in] for out, in in (e, mu): rows e, mu of W Y^T times G
+            Cx<float> He[3], Hm[3], T[2][2];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 He[c] = Y[k][c][0];
@@ -267,27 +467,8 @@ __device__ inline void propagate_rows_mixed(int L, const double* const* gl, cons
                 T[0][b] = e;
                 T[1][b] = m;
             }
-        } else {
-#pragma unroll
-            for (int b = 0; b < 2; b++) {
-                T[0][b] = G[k][0][b];
-                Cx<float> m = cmul(w1, G[k][1][b]);
-                cmac(m, w2, G[k][2][b]);
-                T[1][b] = m;
-            }
+            block_probabilities(T, p[k]);
         }
-        // p[in][out] = |T[out][in]|^2; the third row and column from unitarity
-        float (&pk)[3][3] = p[k];
-#pragma unroll
-        for (int in = 0; in < 2; in++)
-#pragma unroll
-            for (int o = 0; o < 2; o++)
-                pk[in][o] = __builtin_fmaf(T[o][in].re, T[o][in].re, T[o][in].im * T[o][in].im);
-        pk[0][2] = clamp01(1.0f - pk[0][0] - pk[0][1]);
-        pk[1][2] = clamp01(1.0f - pk[1][0] - pk[1][1]);
-        pk[2][0] = clamp01(1.0f - pk[0][0] - pk[1][0]);
-        pk[2][1] = clamp01(1.0f - pk[0][1] - pk[1][1]);
-        pk[2][2] = clamp01(1.0f - pk[2][0] - pk[2][1]);
     }
 }
 

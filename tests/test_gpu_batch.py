@@ -1,6 +1,8 @@
 """Batched parameter scans (one persistent launch over many sets with uneven grids and Earth
-models) against the single-grid mixed propagator (bit for bit: same per-pair code and
-tables) and the oracle (mixed tolerance)."""
+models) against the single-grid mixed propagator and the oracle (mixed tolerance).  The two
+kernels share the per-pair code of rows that cross at most one shell (bit for bit there); for
+deeper rows the single-grid kernel carries the running product in each shell's eigenbasis and the
+batched one as rank-1 updates, so there they agree within the mixed bar."""
 import numpy as np
 import pytest
 
@@ -29,6 +31,16 @@ def small_sets(n=12, seed=21):
     return sets
 
 
+def same_as_single(got, spec):
+    """got (a set's batched results) against the single-grid propagator: bitwise on rows crossing
+    at most one shell, within the mixed bar on the others."""
+    from cudaprob3_amd import metrics as M
+    ref = single(spec)
+    shallow = M.maxlayers_of(spec.cosines, spec.radii, spec.rhos) <= 1
+    np.testing.assert_array_equal(got[:, shallow, :], ref[:, shallow, :])
+    assert np.abs(got - ref).max() <= MIXED_TOL
+
+
 def single(spec):
     with HipPropagatorSingle(0, len(spec.cosines), len(spec.energies), precision="mixed") as p:
         p.setEnergyList(spec.energies)
@@ -49,7 +61,7 @@ def test_batch_matches_single_and_oracle(oracle_mod, graph):
         b.compute()
         for i, s in enumerate(sets):
             got = b.results(i)
-            np.testing.assert_array_equal(got, single(s))
+            same_as_single(got, s)
             ref = oracle_mod.propagate(s.antinu, s.cosines, s.energies, s.radii, s.rhos, s.h_km, s.params)
             assert np.abs(got.transpose(1, 2, 0) - ref).max() <= MIXED_TOL
 
@@ -100,7 +112,7 @@ def test_scan1024_full_size(oracle_mod):
         for i in range(0, 1024, 131):
             s = specs[i]
             got = b.results(i)
-            np.testing.assert_array_equal(got, single(s))
+            same_as_single(got, s)
             rows = np.arange(0, len(s.cosines), 23)
             ref = oracle_mod.propagate(s.antinu, s.cosines[rows], s.energies, s.radii, s.rhos, s.h_km, s.params)
             assert np.abs(got[:, rows, :].transpose(1, 2, 0) - ref).max() <= MIXED_TOL

@@ -1,5 +1,8 @@
 """CPU emulation of the mixed-precision algebra of the GPU kernels (pair_mixed.hpp) against the
-reference's golden vectors: pins the precision design itself, independently of a GPU.
+reference's golden vectors: pins the precision design itself, independently of a GPU.  Two forms
+of the running product are emulated: the rank-1 form of the batched kernel (emulate) and the
+eigenbasis form of the single-grid kernels for rows crossing two or more shells
+(emulate_eigenbasis).
 
 What is emulated, stage by stage (numpy, vectorised over the energies of a cosine row):
   * the real basis (host_util.hpp real_basis): the unitary W = diag(1, W') that makes the vacuum
@@ -214,4 +217,124 @@ def test_emulation_detects_a_wrong_way_out():
     in itself (S_{L-1} ... S_1, the shells in the wrong order) breaks the bar."""
     g = load_golden("prem59_nu")
     bad = emulate(g, way_out=lambda Y: Y)
+    assert np.abs(bad - g["P"]).max() > 1e-2
+
+
+def phase_factor(t):
+    """e^{i phi} for phases t in half turns, as the eigenbasis path forms it: the same FP64
+    reduction and float32 polynomials, cos = sign (1 + (cos z - 1)) and sin = sign sin z."""
+    q = np.rint(t)
+    z = ((t - q) * np.pi).astype(F32)
+    z2 = z * z
+    ps = SIN_C[-1]
+    for c in SIN_C[-2::-1]:
+        ps = z2 * ps + c
+    s0 = (z * z2) * ps + z
+    pc = COS_C[-1]
+    for c in COS_C[-2::-1]:
+        pc = z2 * pc + c
+    c0m1 = z2 * pc
+    sign = np.where(sign_bit(q), F32(-1), F32(1)).astype(F32)
+    return ((sign * c0m1 + sign).astype(F32) + 1j * (sign * s0).astype(F32)).astype(np.complex64)
+
+
+def eigenbasis_tables(Hr, E, rhos, antinu):
+    """Per shell slot: phase rates, v_1, v_2 (float32) and the basis change M_s = O_s^T O_{s-1}
+    (float32 from FP64), O_s = [v_1 x v_2, v_1, v_2] (det +1), O_0 = I in M_1."""
+    tables, prev = [], np.repeat(np.eye(3)[None], len(E), axis=0)
+    for slot, rho in enumerate(rhos):
+        lam, V = spectrum(Hr, E, np.full(len(E), rho), antinu)
+        rate = [(-LOE * (lam[:, k] - lam[:, 0]) / E) / np.pi for k in (1, 2)]
+        v1, v2 = V[:, :, 1], V[:, :, 2]
+        O = np.stack([np.cross(v1, v2), v1, v2], axis=2)          # columns v_0, v_1, v_2
+        M = np.einsum("eak,eac->ekc", O, prev).astype(F32) if slot >= 1 else None
+        tables.append((rate, [v1.astype(F32), v2.astype(F32)], M))
+        if slot >= 1:
+            prev = O
+    return tables
+
+
+def emulate_eigenbasis(g, shuffle_m=False):
+    """The eigenbasis path of the mixed kernels (rows with L >= 2): the running product carried as
+    Z = D_i M_i ... D_1 M_1 (M real 3x3 basis changes, D_i = diag(1, e^{i dphi_1}, e^{i dphi_2})),
+    then Z' = M_L Z and T[x][y] = sum_c (Z' a_x)_c D_L,c (Z' G)_c,y with a_e = e_0, a_mu = row mu
+    of W; L < 2 as in emulate()."""
+    p = Propagator(len(g["cosines"]), len(g["energies"]))
+    p.setEnergyList(g["energies"])
+    p.setCosineList(g["cosines"])
+    p.setDensity(g["radii"], g["rhos"])
+    p.setProductionHeight(float(g["h_km"]))
+    p.setMNSMatrix(*g["params"][:4])
+    p.setNeutrinoMasses(*g["params"][4:])
+    antinu = int(g["antinu"])
+    E = np.asarray(p.energyList, np.float64)
+    n_e = len(E)
+    U18 = np.asarray(p.Mix_U)
+    Hr, W = real_basis((U18[0::2] + 1j * U18[1::2]).reshape(3, 3), np.asarray(p.dm))
+    Wc = np.conj(W[:2, :].T).astype(np.complex64)
+    Wr = W[:2, :].astype(np.complex64)
+    rhos = [0.0] + [float(r) * 0.5 for r in p.rhos]
+    tables = eigenbasis_tables(Hr, E, rhos, antinu)
+    base = emulate(g)
+    out = np.zeros((len(p.cosineList), n_e, 9))
+    for ic, cz in enumerate(p.cosineList):
+        L = int(p.maxlayers[ic])
+        if L < 2:
+            out[ic] = base[ic]
+            continue
+        dist = oracle.layer_distances(p.radii, L, cz, float(p.ProductionHeightinCentimeter)) / 1e5
+
+        def d(i):
+            rate = tables[i][0]
+            return [phase_factor(rate[0] * dist[i]), phase_factor(rate[1] * dist[i])]
+
+        def Mof(i):
+            M = tables[i][2]
+            return M[:, ::-1, :] if shuffle_m else M
+
+        d1 = d(1)
+        Z = Mof(1).astype(np.complex64)
+        Z[:, 1, :] *= d1[0][:, None]
+        Z[:, 2, :] *= d1[1][:, None]
+        for i in range(2, L):
+            Z = (Mof(i) @ Z).astype(np.complex64)
+            di = d(i)
+            Z[:, 1, :] *= di[0][:, None]
+            Z[:, 2, :] *= di[1][:, None]
+        Zp = (Mof(L) @ Z).astype(np.complex64)
+        dL = d(L)
+        G = apply_layer(np.repeat(Wc[None], n_e, axis=0), tables[0][1], [phase_factor_m1(tables[0][0][k] * dist[0])
+                                                                           for k in (0, 1)])
+        Ug = (Zp @ G).astype(np.complex64)                                   # [e][c][y]
+        Ua = (Zp @ Wr.T[None]).astype(np.complex64)                          # [e][c][x]
+        Ua[:, 1, :] *= dL[0][:, None]
+        Ua[:, 2, :] *= dL[1][:, None]
+        T = np.einsum("ecx,ecy->exy", Ua, Ug).astype(np.complex64)           # T[e, out, in]
+        P2 = (np.abs(T) ** 2).astype(F32)
+        P = np.zeros((n_e, 3, 3), F32)
+        P[:, :2, :2] = np.transpose(P2, (0, 2, 1))
+        P[:, 0, 2] = np.clip(1 - P[:, 0, 0] - P[:, 0, 1], 0, 1)
+        P[:, 1, 2] = np.clip(1 - P[:, 1, 0] - P[:, 1, 1], 0, 1)
+        P[:, 2, 0] = np.clip(1 - P[:, 0, 0] - P[:, 1, 0], 0, 1)
+        P[:, 2, 1] = np.clip(1 - P[:, 0, 1] - P[:, 1, 1], 0, 1)
+        P[:, 2, 2] = np.clip(1 - P[:, 2, 0] - P[:, 2, 1], 0, 1)
+        out[ic] = P.reshape(n_e, 9)
+    return out
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_eigenbasis_algebra_meets_tolerance(name):
+    """The eigenbasis form of the running product (rows with L >= 2) against the reference's
+    golden values, within the mixed bar."""
+    g = load_golden(name)
+    P = emulate_eigenbasis(g)
+    err = np.abs(P - g["P"]).max()
+    print(f"{name}: emulated eigenbasis algebra max|dP| {err:.2e}")
+    assert err <= MIXED_TOL
+
+
+def test_eigenbasis_emulation_detects_a_wrong_basis_change():
+    """M_s = O_s^T O_{s-1}: using its rows in the wrong order breaks the bar."""
+    g = load_golden("prem59_nu")
+    bad = emulate_eigenbasis(g, shuffle_m=True)
     assert np.abs(bad - g["P"]).max() > 1e-2
