@@ -95,10 +95,17 @@ hipError_t launch_setup_mixed(const GridDesc& g, const OscParams& p, int ne_pad,
                               double* Rm, hipStream_t s);
 hipError_t launch_spectrum_f64(const GridDesc& g, const OscParams& p, int ne_pad, double* Q, double* R, hipStream_t s);
 
+// Vacuum amplitude constants of the single-grid mixed kernel (a path crossing no shell is one
+// vacuum layer, whose amplitude is delta_xy + sum_{k=1,2} (e^{i dphi_k} - 1) U_xk conj(U_yk)):
+// c[(k - 1) * 8 + (x * 2 + y) * 2 + {0: re, 1: im}] = U_xk conj(U_yk), x, y in (e, mu), float.
+struct VacuumBlock {
+    float c[16];
+};
+
 // Table-driven per-pair kernels.
-// wmu: row 1 of the real basis's W in float (OscParams::Wmu)
+// wmu: row 1 of the real basis's W in float (OscParams::Wmu); vac: the vacuum constants
 hipError_t launch_pairs_mixed(const GridDesc& g, int ne_pad, const double* geom, const float* Q, const float* M,
-                              const double* R, float4 wmu, float* out, hipStream_t s);
+                              const double* R, float4 wmu, const VacuumBlock& vac, float* out, hipStream_t s);
 hipError_t launch_pairs_f64(const GridDesc& g, int ne_pad, const double* geom, const double* Q,
                             const double* R, double* out, hipStream_t s);
 
@@ -113,8 +120,8 @@ hipError_t launch_sum_f64(const double* in, long long n, double* scratch, hipStr
 // its pairs and nine channels, into partials[likelihood_blocks(g)]; nothing else is stored.
 unsigned likelihood_blocks(const GridDesc& g);
 hipError_t launch_pairs_mixed_sqdiff(const GridDesc& g, int ne_pad, const double* geom, const float* Q,
-                                     const float* M, const double* R, float4 wmu, const float* data,
-                                     double* partials, hipStream_t s);
+                                     const float* M, const double* R, float4 wmu, const VacuumBlock& vac,
+                                     const float* data, double* partials, hipStream_t s);
 // sum_i (a_i - b_i)^2 in FP64 for device floats; result in partials[reduce_partials_needed()].
 hipError_t launch_sqdiff_sum(const float* a, const float* b, std::size_t n, double* partials, hipStream_t s);
 

@@ -326,10 +326,12 @@ int enqueue_pairs(cp3h_propagator* h, const cp3::GridDesc& g, const float* data)
     } else if (h->precision == CP3H_MIXED) {
         if (data)
             CP3H_HIP(cp3::launch_pairs_mixed_sqdiff(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_M,
-                                                    h->d_R, wmu_f32(h->params), data, h->d_like, h->stream));
+                                                    h->d_R, wmu_f32(h->params), cp3host::vacuum_block(h->params.U), data, h->d_like,
+                                                    h->stream));
         else
             CP3H_HIP(cp3::launch_pairs_mixed(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_M,
-                                             h->d_R, wmu_f32(h->params), static_cast<float*>(h->d_out), h->stream));
+                                             h->d_R, wmu_f32(h->params), cp3host::vacuum_block(h->params.U),
+                                             static_cast<float*>(h->d_out), h->stream));
     } else {
         CP3H_HIP(cp3::launch_pairs_f64(g, h->ne_pad, h->d_geom, static_cast<const double*>(h->d_Q), h->d_R,
                                        static_cast<double*>(h->d_out), h->stream));

@@ -115,6 +115,6 @@ extern "C" int cp3h_propagate_mixed_noalloc(int device, const double* cosines_de
                           n_cosines, n_energies, n_rows, production_height_cm};
     hipStream_t s = static_cast<hipStream_t>(stream);
     CP3H_HIP(cp3::launch_setup_mixed(g, p, int(ne_pad), geom, Q, M, R, s));
-    CP3H_HIP(cp3::launch_pairs_mixed(g, int(ne_pad), geom, Q, M, R, wmu, out_device, s));
+    CP3H_HIP(cp3::launch_pairs_mixed(g, int(ne_pad), geom, Q, M, R, wmu, cp3host::vacuum_block(p.U), out_device, s));
     return CP3H_OK;
 }

@@ -33,60 +33,57 @@ __device__ inline Cx<float> cmul(Cx<float> a, Cx<float> b) {
     return r;
 }
 
-// e^{i phi} - 1 for a phase given in half turns (phi = t * pi), as (cos phi - 1, sin phi).
-// Cody-Waite style reduction in FP64: q = rint(t), r = t - q in [-1/2, 1/2] is exact, so the
-// phase reaches FP32 only as the remainder z = r * pi in [-pi/2, pi/2]; there sin z and
-// cos z - 1 are polynomials evaluated in FP32 by Horner's rule, and e^{i q pi} = (-1)^q flips
-// the sign.  The coefficients are least-squares fits to sin and cos on [-pi/2, pi/2] with
-// Lawson's iterative reweighting towards the minimax error: |sin z - s0| <= 4.6e-9 and
+// Cody-Waite style reduction of a phase of t half turns (phi = t pi) in FP64: q = rint(t),
+// r = t - q in [-1/2, 1/2] is exact, so the phase reaches FP32 only as the remainder
+// z = r pi in [-pi/2, pi/2]; there sin z and cos z - 1 are polynomials evaluated in FP32 by
+// Horner's rule (sin z with its leading term z added last, exactly), and e^{i q pi} = (-1)^q is
+// returned as a sign.  The coefficients are least-squares fits to sin and cos on [-pi/2, pi/2]
+// with Lawson's iterative reweighting towards the minimax error: |sin z - s0| <= 4.6e-9 and
 // |cos z - 1 - c0m1| <= 2.4e-10 before FP32 rounding (Taylor polynomials of the same degrees
-// would leave 3.6e-6).
+// would leave 3.6e-6).  Folding pi into the coefficients (polynomials in r) would save the FP64
+// multiply but puts pi's float rounding into the leading term of sin: |e^{i phi}|^2 - 1 then
+// averages -3e-8 instead of 4e-9, a bias that grows linearly over the ~200 layer products of a
+// core-crossing path on a 100-shell Earth.
+__device__ inline __attribute__((always_inline)) void reduced_phase(double t, float& s0, float& c0m1, float& sign) {
+#pragma clang fp contract(off)
+    const double q = rint(t);
+    const float z = float((t - q) * 3.141592653589793);
+    const float z2 = z * z;
+    float ps = __builtin_fmaf(z2, 2.600053676e-6f, -1.980661473e-4f);
+    ps = __builtin_fmaf(z2, ps, 8.333017118e-3f);
+    ps = __builtin_fmaf(z2, ps, -1.666665673e-1f);
+    s0 = __builtin_fmaf(z * z2, ps, z);
+    float pc = __builtin_fmaf(z2, -2.607709746e-7f, 2.476188638e-5f);
+    pc = __builtin_fmaf(z2, pc, -1.388840377e-3f);
+    pc = __builtin_fmaf(z2, pc, 4.166664183e-2f);
+    pc = __builtin_fmaf(z2, pc, -0.5f);
+    c0m1 = z2 * pc;
+    // the parity of q is the last mantissa bit of q + 1.5 * 2^52: for |q| < 2^51 the sum lies in
+    // [2^52, 2^53), where the last bit is worth 1, and the added even integer keeps the parity
+    // (2^52 alone would not do for q < 0: the sum then falls below 2^52, where the last bit is
+    // worth 1/2).  No 32-bit integer conversion, so phases up to 2^51 half turns are handled.
+    const unsigned lo = unsigned(__builtin_bit_cast(unsigned long long, q + 6755399441055744.0));
+    sign = __builtin_bit_cast(float, (lo << 31) | 0x3f800000u);
+}
+
+// e^{i phi} - 1 for a phase given in half turns, as (cos phi - 1, sin phi): with the sign,
+// (c0m1, s0) for even q and (-c0m1 - 2, -s0) for odd q, each with one rounding.
 __device__ inline void phase_factor_m1(double t, float& cm1, float& sn) {
     // every fused multiply-add is written out and nothing else may be fused, so that each
     // inlined copy (one row, two rows, batched) rounds identically
 #pragma clang fp contract(off)
-    const double q = rint(t);
-    const float z = float((t - q) * 3.141592653589793);
-    const float z2 = z * z;
-    float ps = __builtin_fmaf(z2, 2.600053676e-6f, -1.980661473e-4f);
-    ps = __builtin_fmaf(z2, ps, 8.333017118e-3f);
-    ps = __builtin_fmaf(z2, ps, -1.666665673e-1f);
-    const float s0 = __builtin_fmaf(z * z2, ps, z);
-    float pc = __builtin_fmaf(z2, -2.607709746e-7f, 2.476188638e-5f);
-    pc = __builtin_fmaf(z2, pc, -1.388840377e-3f);
-    pc = __builtin_fmaf(z2, pc, 4.166664183e-2f);
-    pc = __builtin_fmaf(z2, pc, -0.5f);
-    const float c0m1 = z2 * pc;
-    // sign = (-1)^q: e^{i phi} - 1 = sign (1 + c0m1 + i s0) - 1, i.e. (c0m1, s0) for even q and
-    // (-c0m1 - 2, -s0) for odd q, each with one rounding.  The parity of q is the last mantissa
-    // bit of q + 1.5 * 2^52: for |q| < 2^51 the sum lies in [2^52, 2^53), where the last bit is
-    // worth 1, and the added even integer keeps the parity (2^52 alone would not do for q < 0:
-    // the sum then falls below 2^52, where the last bit is worth 1/2).  No 32-bit integer
-    // conversion, so phases up to 2^51 half turns are handled.
-    const unsigned lo = unsigned(__builtin_bit_cast(unsigned long long, q + 6755399441055744.0));
-    const float sign = __builtin_bit_cast(float, (lo << 31) | 0x3f800000u);
+    float s0, c0m1, sign;
+    reduced_phase(t, s0, c0m1, sign);
     cm1 = __builtin_fmaf(sign, c0m1, sign - 1.0f);
     sn = sign * s0;
 }
 
-// e^{i phi} for a phase in half turns: the reduction and polynomials of phase_factor_m1, with
-// cos phi = sign (1 + (cos z - 1)) and sin phi = sign sin z, each with one rounding.
+// e^{i phi} for a phase in half turns: cos phi = sign (1 + (cos - 1)) and sin phi = sign sin,
+// each with one rounding.
 __device__ inline void phase_factor(double t, float& c, float& sn) {
 #pragma clang fp contract(off)
-    const double q = rint(t);
-    const float z = float((t - q) * 3.141592653589793);
-    const float z2 = z * z;
-    float ps = __builtin_fmaf(z2, 2.600053676e-6f, -1.980661473e-4f);
-    ps = __builtin_fmaf(z2, ps, 8.333017118e-3f);
-    ps = __builtin_fmaf(z2, ps, -1.666665673e-1f);
-    const float s0 = __builtin_fmaf(z * z2, ps, z);
-    float pc = __builtin_fmaf(z2, -2.607709746e-7f, 2.476188638e-5f);
-    pc = __builtin_fmaf(z2, pc, -1.388840377e-3f);
-    pc = __builtin_fmaf(z2, pc, 4.166664183e-2f);
-    pc = __builtin_fmaf(z2, pc, -0.5f);
-    const float c0m1 = z2 * pc;
-    const unsigned lo = unsigned(__builtin_bit_cast(unsigned long long, q + 6755399441055744.0));
-    const float sign = __builtin_bit_cast(float, (lo << 31) | 0x3f800000u);
+    float s0, c0m1, sign;
+    reduced_phase(t, s0, c0m1, sign);
     c = __builtin_fmaf(sign, c0m1, sign);
     sn = sign * s0;
 }
@@ -295,7 +292,8 @@ __device__ inline __attribute__((always_inline)) void block_probabilities(const 
 //  * rank-1 (batched kernel, whose tables then need no basis changes): Y += v_k (w_k v_k^T Y)
 //    for k = 1, 2 per shell (w_k = e^{i dphi_k} - 1, sum_k v_k v_k^T = I), then
 //    T = (W Y^T)[e mu, :] S_L (Y G).
-// L <= 1: T = W S_1 G (S_1 as two rank-1 updates of G) or W G.
+// L <= 1: T = W S_1 G (S_1 as two rank-1 updates of G) or W G; in the single-grid kernel a path
+// crossing no shell takes the vacuum form T = U D_0 U^dagger with host constants U_xk conj(U_yk).
 //
 // kRows cosine rows with the same L are propagated together at this lane's energy: they cross
 // the same shells, so each shell's table entries are loaded once for all of them (only the
@@ -305,13 +303,45 @@ __device__ inline __attribute__((always_inline)) void block_probabilities(const 
 template <int kRows, bool kEigen>
 __device__ inline void propagate_rows_mixed(int L, const double* const* gl, const float4* __restrict__ Q,
                                             const float4* __restrict__ M, const double2* __restrict__ R,
-                                            std::size_t st, int ie, float4 wmu, float (*p)[3][3]) {
+                                            std::size_t st, int ie, float4 wmu, const VacuumBlock* vac,
+                                            float (*p)[3][3]) {
 #pragma clang fp contract(off)
     static_assert(sizeof(float4) == 16 && sizeof(double2) == 16, "table entries are 16 bytes");
     const unsigned off = unsigned(ie) * 16u;
     const Cx<float> w1{wmu.x, wmu.y}, w2{wmu.z, wmu.w};
     LayerTable t;
     Cx<float> G[kRows][3][2];   // columns e, mu of S_0 W^dagger
+    if (kEigen && L == 0) {
+        // one vacuum layer: T[x][y] = delta_xy + sum_{k=1,2} w_k U_xk conj(U_yk), w_k = e^{i dphi_k} - 1
+        // (W S_0 W^dagger = U D U^dagger in vacuum); only the phase rates are read from the table
+        const double2 r = *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(R) + off);
+#pragma unroll
+        for (int k = 0; k < kRows; k++) {
+            LayerPhases w;
+            phase_factor_m1(r.x * gl[k][0], w.re[0], w.im[0]);
+            phase_factor_m1(r.y * gl[k][0], w.re[1], w.im[1]);
+            Cx<float> T[2][2];
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+#pragma unroll
+                for (int y = 0; y < 2; y++) {
+                    const float* c1 = vac->c + (x * 2 + y) * 2;
+                    const float* c2 = c1 + 8;
+                    float re = x == y ? 1.0f : 0.0f, im = 0.0f;
+                    re = __builtin_fmaf(w.re[0], c1[0], re);
+                    re = __builtin_fmaf(-w.im[0], c1[1], re);
+                    re = __builtin_fmaf(w.re[1], c2[0], re);
+                    re = __builtin_fmaf(-w.im[1], c2[1], re);
+                    im = __builtin_fmaf(w.re[0], c1[1], im);
+                    im = __builtin_fmaf(w.im[0], c1[0], im);
+                    im = __builtin_fmaf(w.re[1], c2[1], im);
+                    im = __builtin_fmaf(w.im[1], c2[0], im);
+                    T[x][y] = Cx<float>{re, im};
+                }
+            block_probabilities(T, p[k]);
+        }
+        return;
+    }
     if (L < 2) {
         load_table(Q, R, st, off, 0, t);
 #pragma unroll

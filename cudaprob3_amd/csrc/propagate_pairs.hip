@@ -184,6 +184,7 @@ struct MixedArgs {
     const float4* M;          // [(rows + 1) * kMixedMGroups][ne_pad]
     const double2* R;         // [(rows + 1)][ne_pad]
     float4 wmu;               // row mu of the real basis's W
+    VacuumBlock vac;          // U_xk conj(U_yk) for the paths that cross no shell
     float* out;               // SoA [9][n_c][n_e]
     const float* data;        // likelihood variant: SoA [9][n_c][n_e] reference values
     double* partials;         // likelihood variant: one sum per workgroup
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void pairs_mixed_kernel(MixedArgs a) {
     if (n_rows == 2 && crossed_shells(a.maxlayers, ic0 + 1, a.rows) == L0) {
         const double* gl[2] = {a.geom + std::size_t(ic0) * (a.rows + 1), a.geom + std::size_t(ic0 + 1) * (a.rows + 1)};
         float p[2][3][3];
-        propagate_rows_mixed<2, true>(L0, gl, a.Q, a.M, a.R, st, ie, a.wmu, p);
+        propagate_rows_mixed<2, true>(L0, gl, a.Q, a.M, a.R, st, ie, a.wmu, &a.vac, p);
         emit(ic0, p[0]);
         emit(ic0 + 1, p[1]);
     } else {
@@ -244,7 +245,8 @@ __global__ __launch_bounds__(kBlock) void pairs_mixed_kernel(MixedArgs a) {
             const int ic = ic0 + k;
             const double* gl[1] = {a.geom + std::size_t(ic) * (a.rows + 1)};
             float p[1][3][3];
-            propagate_rows_mixed<1, true>(crossed_shells(a.maxlayers, ic, a.rows), gl, a.Q, a.M, a.R, st, ie, a.wmu, p);
+            propagate_rows_mixed<1, true>(crossed_shells(a.maxlayers, ic, a.rows), gl, a.Q, a.M, a.R, st, ie, a.wmu,
+                                          &a.vac, p);
             emit(ic, p[0]);
         }
     }
@@ -275,20 +277,20 @@ unsigned likelihood_blocks(const GridDesc& g) {
 }
 
 hipError_t launch_pairs_mixed(const GridDesc& g, int ne_pad, const double* geom, const float* Q, const float* M,
-                              const double* R, float4 wmu, float* out, hipStream_t s) {
+                              const double* R, float4 wmu, const VacuumBlock& vac, float* out, hipStream_t s) {
     if (g.n_c == 0 || g.n_e == 0) return hipSuccess;
     MixedArgs a{g.maxlayers, geom, reinterpret_cast<const float4*>(Q), reinterpret_cast<const float4*>(M),
-                reinterpret_cast<const double2*>(R), wmu, out, nullptr, nullptr, g.n_c, g.n_e, g.rows, ne_pad};
+                reinterpret_cast<const double2*>(R), wmu, vac, out, nullptr, nullptr, g.n_c, g.n_e, g.rows, ne_pad};
     hipLaunchKernelGGL(pairs_mixed_kernel<false>, dim3(likelihood_blocks(g)), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_pairs_mixed_sqdiff(const GridDesc& g, int ne_pad, const double* geom, const float* Q,
-                                     const float* M, const double* R, float4 wmu, const float* data,
-                                     double* partials, hipStream_t s) {
+                                     const float* M, const double* R, float4 wmu, const VacuumBlock& vac,
+                                     const float* data, double* partials, hipStream_t s) {
     if (g.n_c == 0 || g.n_e == 0) return hipSuccess;
     MixedArgs a{g.maxlayers, geom, reinterpret_cast<const float4*>(Q), reinterpret_cast<const float4*>(M),
-                reinterpret_cast<const double2*>(R), wmu, nullptr, data, partials, g.n_c, g.n_e, g.rows, ne_pad};
+                reinterpret_cast<const double2*>(R), wmu, vac, nullptr, data, partials, g.n_c, g.n_e, g.rows, ne_pad};
     hipLaunchKernelGGL(pairs_mixed_kernel<true>, dim3(likelihood_blocks(g)), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }

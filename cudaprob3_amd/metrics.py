@@ -3,16 +3,23 @@
 Two counts, both add/sub/mul = 1 and FMA = 2 flops, conversions, selects, rounding (rint),
 min/max and bit operations not counted:
 
-* The algorithm this build's mixed-precision pair kernel runs (``mixed_flops_per_pair``,
+* The algorithm this build's mixed-precision pair kernels run (``mixed_flops_per_pair``,
   pair_mixed.hpp): per layer two phase factors (FP64: t = r*L, t - q, (t - q)*pi, the parity
-  add; FP32: 23 flops of polynomials and sign), per shell of the way in one update of the 3x3
-  running product by two real rank-1 terms (168), and once per pair the first shell (54), the
-  atmosphere (76), the product with the atmosphere columns (132), the innermost shell on two
-  columns (112), the 2x2 block (130; 28 without a running product) and the nine probabilities
-  (22).  For L crossed shells:
-      FP32: L = 0: 172, L = 1: 330, L >= 2: 214 L + 236;   FP64: 8 (L + 1).
-  This is the roofline's numerator: it agrees within about 1 % with the FP32/FP64 flops that
-  rocprofv3's SQ_INSTS_VALU_{ADD,MUL,FMA}_F{32,64} counters report for the kernel.
+  add; FP32: 22 flops of polynomials and sign for e^{i phi}, 23 for e^{i phi} - 1), the nine
+  probabilities from the 2x2 block (22), and for L crossed shells:
+    - L = 0: single-grid kernels: the vacuum block from the constants U_xk conj(U_yk) (64);
+      batched kernel: the atmosphere columns G (76) and the block from G (28);
+    - L = 1: G (76), the innermost shell on G as two rank-1 terms (112) and the block (28);
+    - L >= 2, eigenbasis form (single-grid kernels): the first shell D_1 M_1 (12), per further
+      shell of the way in one basis change and phase scaling (126), the innermost basis change
+      (90), G (76), h = D_L Z' a_x (66), u = Z' G (132) and the block (88);
+    - L >= 2, rank-1 form (batched kernel): the first shell (54), per further shell two rank-1
+      terms on the 3x3 running product (168), G (76), Y G (132), the innermost shell on two
+      columns (112) and the block through W Y^T (130).
+  FP32 per pair: L = 0: 132 (single-grid) or 172 (batched), L = 1: 330, L >= 2: 170 L + 280
+  (eigenbasis) or 214 L + 236 (rank-1); FP64: 8 (L + 1).  This is the roofline's numerator; it
+  agrees within about 1 % with the FP32/FP64 flops that rocprofv3's
+  SQ_INSTS_VALU_{ADD,MUL,FMA}_F{32,64} counters report.
 * The reference algorithm as written (physics.hpp:268-702), kept as a measure of the work the
   reference would do for the same results (``reference_work``): one layer step
   (get_transition_matrix) = getMfast 69 + get_product 788 + getA 874 = 1731 flops (plus 2 sqrt,
@@ -38,35 +45,51 @@ PEAK_FP64_VALU_TFLOPS = 78.6       # vector FP64
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_MFMA_TFLOPS = 2500.0
 
-# pieces of the mixed kernel's count (pair_mixed.hpp), per pair
+# pieces of the mixed kernels' count (pair_mixed.hpp), per pair
 MIXED_PHASE_F32 = 23          # one phase factor e^{i phi} - 1: polynomials, sign
+MIXED_DIAG_F32 = 22           # one phase factor e^{i phi} (eigenbasis form)
 MIXED_PHASE_F64 = 4           # t = r L, t - q, (t - q) pi, q + 1.5 2^52
-MIXED_SHELL_UPDATE = 168      # 3x3 running product, two rank-1 terms (apply_layer<3>)
-MIXED_FIRST_SHELL = 54        # init_layer
 MIXED_ATMOSPHERE = 76         # atmosphere columns G
-MIXED_Y_TIMES_G = 132         # running product times G
-MIXED_INNERMOST = 112         # innermost shell on two columns (apply_layer<2>)
-MIXED_BLOCK = 130             # the 2x2 block through W Y^T
-MIXED_BLOCK_SHALLOW = 28      # the 2x2 block for L < 2
 MIXED_PROBS = 22              # four |T|^2 and five unitarity closures
+MIXED_BLOCK_SHALLOW = 28      # the 2x2 block from G for L < 2
+MIXED_INNERMOST = 112         # a shell on two columns as two rank-1 terms (apply_layer<2>)
+MIXED_VACUUM_BLOCK = 64       # L = 0, single-grid: delta + sum_k w_k U_xk conj(U_yk) for four entries
+# eigenbasis form (L >= 2)
+EIGEN_FIRST_SHELL = 12        # Z = D_1 M_1
+EIGEN_SHELL = 126             # Z <- D_i M_i Z
+EIGEN_INNERMOST = 90          # Z' = M_L Z
+EIGEN_TAIL = 66 + 132 + 88    # h = D_L Z' a_x, u = Z' G, the 2x2 block
+# rank-1 form (L >= 2)
+RANK1_FIRST_SHELL = 54        # init_layer
+RANK1_SHELL = 168             # 3x3 running product, two rank-1 terms (apply_layer<3>)
+RANK1_Y_TIMES_G = 132         # running product times G
+RANK1_BLOCK = 130             # the 2x2 block through W Y^T
 LIKELIHOOD_F64_PER_PAIR = 27  # nine (P - D)^2 accumulated in FP64 (likelihood variant)
 
 
-def mixed_flops_per_pair(L):
-    """(FP32, FP64) flops of the mixed pair kernel for a pair crossing L shells."""
+def mixed_flops_per_pair(L, eigenbasis: bool = True):
+    """(FP32, FP64) flops of the mixed pair kernels for a pair crossing L shells: the single-grid
+    kernels (eigenbasis form) or the batched kernel (rank-1 form)."""
     L = np.asarray(L, dtype=np.int64)
-    phases = 2 * (L + 1)
-    f32 = phases * MIXED_PHASE_F32 + MIXED_ATMOSPHERE + MIXED_PROBS
-    f32 = f32 + np.where(L >= 1, MIXED_INNERMOST, 0)
-    f32 = f32 + np.where(L >= 2, MIXED_FIRST_SHELL + MIXED_SHELL_UPDATE * (L - 2) + MIXED_Y_TIMES_G + MIXED_BLOCK,
-                         MIXED_BLOCK_SHALLOW)
-    f64 = phases * MIXED_PHASE_F64
+    shallow = (2 * MIXED_PHASE_F32 + MIXED_ATMOSPHERE + MIXED_BLOCK_SHALLOW + MIXED_PROBS +
+               np.where(L == 1, 2 * MIXED_PHASE_F32 + MIXED_INNERMOST, 0))
+    if eigenbasis:
+        shallow = np.where(L == 0, 2 * MIXED_PHASE_F32 + MIXED_VACUUM_BLOCK + MIXED_PROBS, shallow)
+    if eigenbasis:
+        deep = (2 * MIXED_PHASE_F32 + 2 * MIXED_DIAG_F32 * L + EIGEN_FIRST_SHELL + EIGEN_SHELL * (L - 2) +
+                EIGEN_INNERMOST + MIXED_ATMOSPHERE + EIGEN_TAIL + MIXED_PROBS)
+    else:
+        deep = (2 * MIXED_PHASE_F32 * (L + 1) + RANK1_FIRST_SHELL + RANK1_SHELL * (L - 2) + MIXED_ATMOSPHERE +
+                RANK1_Y_TIMES_G + MIXED_INNERMOST + RANK1_BLOCK + MIXED_PROBS)
+    f32 = np.where(L >= 2, deep, shallow)
+    f64 = 2 * MIXED_PHASE_F64 * (L + 1)
     return f32, f64
 
 
-def mixed_grid_flops(maxlayers, n_energies: int, likelihood: bool = False) -> tuple[float, float]:
+def mixed_grid_flops(maxlayers, n_energies: int, likelihood: bool = False,
+                     eigenbasis: bool = True) -> tuple[float, float]:
     """(FP32, FP64) flops of one mixed pair-kernel launch over a grid."""
-    f32, f64 = mixed_flops_per_pair(maxlayers)
+    f32, f64 = mixed_flops_per_pair(maxlayers, eigenbasis)
     n_pairs = len(np.asarray(maxlayers)) * n_energies
     extra = LIKELIHOOD_F64_PER_PAIR * n_pairs if likelihood else 0
     return float(f32.sum()) * n_energies, float(f64.sum()) * n_energies + extra

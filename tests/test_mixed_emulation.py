@@ -80,8 +80,9 @@ def test_sign_bit_is_the_parity():
     assert not np.array_equal(wrong, q.astype(np.int64) & 1)
 
 
-def phase_factor_m1(t):
-    """e^{i phi} - 1 for phases t in half turns: FP64 reduction, float32 polynomials, sign (-1)^q."""
+def reduced_phase(t):
+    """(sin z, cos z - 1, (-1)^q) of phases t in half turns as the kernels form them: FP64
+    reduction q = rint(t), z = (t - q) pi rounded to float32, float32 polynomials in z."""
     q = np.rint(t)
     z = ((t - q) * np.pi).astype(F32)
     z2 = z * z
@@ -94,6 +95,12 @@ def phase_factor_m1(t):
         pc = z2 * pc + c
     c0m1 = z2 * pc
     sign = np.where(sign_bit(q), F32(-1), F32(1)).astype(F32)
+    return s0, c0m1, sign
+
+
+def phase_factor_m1(t):
+    """e^{i phi} - 1 for phases t in half turns: FP64 reduction, float32 polynomials, sign (-1)^q."""
+    s0, c0m1, sign = reduced_phase(t)
     cm1 = sign * c0m1 + (sign - F32(1))
     sn = sign * s0
     return (cm1.astype(F32) + 1j * sn.astype(F32)).astype(np.complex64)
@@ -222,19 +229,8 @@ def test_emulation_detects_a_wrong_way_out():
 
 def phase_factor(t):
     """e^{i phi} for phases t in half turns, as the eigenbasis path forms it: the same FP64
-    reduction and float32 polynomials, cos = sign (1 + (cos z - 1)) and sin = sign sin z."""
-    q = np.rint(t)
-    z = ((t - q) * np.pi).astype(F32)
-    z2 = z * z
-    ps = SIN_C[-1]
-    for c in SIN_C[-2::-1]:
-        ps = z2 * ps + c
-    s0 = (z * z2) * ps + z
-    pc = COS_C[-1]
-    for c in COS_C[-2::-1]:
-        pc = z2 * pc + c
-    c0m1 = z2 * pc
-    sign = np.where(sign_bit(q), F32(-1), F32(1)).astype(F32)
+    reduction and float32 polynomials, cos = sign (1 + (cos - 1)) and sin = sign sin."""
+    s0, c0m1, sign = reduced_phase(t)
     return ((sign * c0m1 + sign).astype(F32) + 1j * (sign * s0).astype(F32)).astype(np.complex64)
 
 
@@ -255,10 +251,11 @@ def eigenbasis_tables(Hr, E, rhos, antinu):
 
 
 def emulate_eigenbasis(g, shuffle_m=False):
-    """The eigenbasis path of the mixed kernels (rows with L >= 2): the running product carried as
+    """The single-grid mixed kernel: for rows with L >= 2 the running product carried as
     Z = D_i M_i ... D_1 M_1 (M real 3x3 basis changes, D_i = diag(1, e^{i dphi_1}, e^{i dphi_2})),
     then Z' = M_L Z and T[x][y] = sum_c (Z' a_x)_c D_L,c (Z' G)_c,y with a_e = e_0, a_mu = row mu
-    of W; L < 2 as in emulate()."""
+    of W; for L = 0 the vacuum form T = I + sum_{k=1,2} w_k U_xk conj(U_yk); L = 1 as in
+    emulate()."""
     p = Propagator(len(g["cosines"]), len(g["energies"]))
     p.setEnergyList(g["energies"])
     p.setCosineList(g["cosines"])
@@ -276,13 +273,32 @@ def emulate_eigenbasis(g, shuffle_m=False):
     rhos = [0.0] + [float(r) * 0.5 for r in p.rhos]
     tables = eigenbasis_tables(Hr, E, rhos, antinu)
     base = emulate(g)
+    # vacuum constants (W v_k)_x conj((W v_k)_y) with the table's labelling of the eigenvectors (for
+    # the kernel's labelling, by mass index, these are U_xk conj(U_yk))
+    _, V0 = spectrum(Hr, E[:1], np.zeros(1), antinu)
+    wv = [(W @ V0[0, :, k])[:2] for k in (1, 2)]
+    vac = [np.outer(x, np.conj(x)).astype(np.complex64) for x in wv]
     out = np.zeros((len(p.cosineList), n_e, 9))
     for ic, cz in enumerate(p.cosineList):
         L = int(p.maxlayers[ic])
+        dist = oracle.layer_distances(p.radii, L, cz, float(p.ProductionHeightinCentimeter)) / 1e5
+        if L == 0:       # one vacuum layer: T = I + sum_k w_k U_xk conj(U_yk)
+            w = [phase_factor_m1(tables[0][0][k] * dist[0]) for k in (0, 1)]
+            T = (np.eye(2, dtype=np.complex64)[None] + w[0][:, None, None] * vac[0][None] +
+                 w[1][:, None, None] * vac[1][None]).astype(np.complex64)
+            P2 = (np.abs(T) ** 2).astype(F32)
+            P = np.zeros((n_e, 3, 3), F32)
+            P[:, :2, :2] = np.transpose(P2, (0, 2, 1))
+            P[:, 0, 2] = np.clip(1 - P[:, 0, 0] - P[:, 0, 1], 0, 1)
+            P[:, 1, 2] = np.clip(1 - P[:, 1, 0] - P[:, 1, 1], 0, 1)
+            P[:, 2, 0] = np.clip(1 - P[:, 0, 0] - P[:, 1, 0], 0, 1)
+            P[:, 2, 1] = np.clip(1 - P[:, 0, 1] - P[:, 1, 1], 0, 1)
+            P[:, 2, 2] = np.clip(1 - P[:, 2, 0] - P[:, 2, 1], 0, 1)
+            out[ic] = P.reshape(n_e, 9)
+            continue
         if L < 2:
             out[ic] = base[ic]
             continue
-        dist = oracle.layer_distances(p.radii, L, cz, float(p.ProductionHeightinCentimeter)) / 1e5
 
         def d(i):
             rate = tables[i][0]
