@@ -63,6 +63,13 @@ float4 wmu_f32(const cp3::OscParams& p) {
     return make_float4(float(p.Wmu[0]), float(p.Wmu[1]), float(p.Wmu[2]), float(p.Wmu[3]));
 }
 
+// the single-grid mixed kernel's vacuum constants for the handle's mixing matrix
+cp3::VacuumBlock vacuum_of(const cp3::OscParams& p) {
+    cp3::VacuumBlock b{};
+    cp3host::vacuum_block(p.U, b.c);
+    return b;
+}
+
 int sync_and_select(cp3h_propagator* h) {
     CP3H_HIP(hipSetDevice(h->device));
     CP3H_HIP(hipStreamSynchronize(h->stream));
@@ -326,11 +333,11 @@ int enqueue_pairs(cp3h_propagator* h, const cp3::GridDesc& g, const float* data)
     } else if (h->precision == CP3H_MIXED) {
         if (data)
             CP3H_HIP(cp3::launch_pairs_mixed_sqdiff(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_M,
-                                                    h->d_R, wmu_f32(h->params), cp3host::vacuum_block(h->params.U), data, h->d_like,
+                                                    h->d_R, wmu_f32(h->params), vacuum_of(h->params), data, h->d_like,
                                                     h->stream));
         else
             CP3H_HIP(cp3::launch_pairs_mixed(g, h->ne_pad, h->d_geom, static_cast<const float*>(h->d_Q), h->d_M,
-                                             h->d_R, wmu_f32(h->params), cp3host::vacuum_block(h->params.U),
+                                             h->d_R, wmu_f32(h->params), vacuum_of(h->params),
                                              static_cast<float*>(h->d_out), h->stream));
     } else {
         CP3H_HIP(cp3::launch_pairs_f64(g, h->ne_pad, h->d_geom, static_cast<const double*>(h->d_Q), h->d_R,

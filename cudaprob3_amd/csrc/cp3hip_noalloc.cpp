@@ -106,6 +106,8 @@ extern "C" int cp3h_propagate_mixed_noalloc(int device, const double* cosines_de
     cp3host::real_basis(p.U, p.DM, p.Hr, p.Wmu);
     p.antinu = antineutrino;
     const float4 wmu = make_float4(float(p.Wmu[0]), float(p.Wmu[1]), float(p.Wmu[2]), float(p.Wmu[3]));
+    cp3::VacuumBlock vac{};
+    cp3host::vacuum_block(p.U, vac.c);
 
     double* geom = reinterpret_cast<double*>(base);
     float* Q = reinterpret_cast<float*>(base + geom_bytes);
@@ -115,6 +117,6 @@ extern "C" int cp3h_propagate_mixed_noalloc(int device, const double* cosines_de
                           n_cosines, n_energies, n_rows, production_height_cm};
     hipStream_t s = static_cast<hipStream_t>(stream);
     CP3H_HIP(cp3::launch_setup_mixed(g, p, int(ne_pad), geom, Q, M, R, s));
-    CP3H_HIP(cp3::launch_pairs_mixed(g, int(ne_pad), geom, Q, M, R, wmu, cp3host::vacuum_block(p.U), out_device, s));
+    CP3H_HIP(cp3::launch_pairs_mixed(g, int(ne_pad), geom, Q, M, R, wmu, vac, out_device, s));
     return CP3H_OK;
 }

@@ -2,7 +2,6 @@
 #pragma once
 
 #include "../../include/cp3hip.h"
-#include "common.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -114,19 +113,17 @@ inline void real_basis(const double* U18, const double* DM9, double Hr[6], doubl
 }
 
 // The single-grid mixed kernel's vacuum constants U_xk conj(U_yk) (k = 1, 2; x, y in (e, mu)),
-// from the caller's U in FP64, rounded to float.
-inline cp3::VacuumBlock vacuum_block(const double* U18) {
-    cp3::VacuumBlock b{};
+// from the caller's U in FP64, rounded to float: c[(k - 1) * 8 + (x * 2 + y) * 2 + {0: re, 1: im}].
+inline void vacuum_block(const double* U18, float c[16]) {
     for (int k = 1; k < 3; k++)
         for (int x = 0; x < 2; x++)
             for (int y = 0; y < 2; y++) {
                 const double ar = U18[2 * (3 * x + k)], ai = U18[2 * (3 * x + k) + 1];
                 const double br = U18[2 * (3 * y + k)], bi = U18[2 * (3 * y + k) + 1];
                 const int i = (k - 1) * 8 + (x * 2 + y) * 2;
-                b.c[i] = float(ar * br + ai * bi);
-                b.c[i + 1] = float(ai * br - ar * bi);
+                c[i] = float(ar * br + ai * bi);
+                c[i + 1] = float(ai * br - ar * bi);
             }
-    return b;
 }
 
 }  // namespace cp3host

@@ -155,15 +155,15 @@ const char* cp3h_last_error(void);
    forbid hipMalloc).  It is cp3h_compute_async of a CP3H_MIXED handle without the handle: the
    grid, the Earth model and the results are caller device memory, the parameters are host
    arrays as for cp3h_set_params_f64, and the kernels' scratch (layer lengths and the spectrum
-   table) lives in one static device array of 64 MiB per GPU owned by the library.
+   table) lives in one static device array of 128 MiB per GPU owned by the library.
      cosines_device[n_cosines], maxlayers_device[n_cosines] (each in [0, n_rows]; entries outside
      are clamped into it, so no read leaves the arrays, but such rows' results are meaningless),
      energies_device[n_energies] (GeV), radii_desc_device / rhos_desc_device[n_rows] (descending,
      as for cp3h_set_density_f64), out_device: float SoA [9][n_cosines][n_energies].
    Enqueues two kernels on `stream` (a hipStream_t as void*, NULL = the null stream) and returns;
    calls no hipMalloc, hipFree or other allocating routine.  Limit: 8 n_cosines (n_rows + 1) +
-   48 (n_rows + 1) n_energies_padded bytes <= 64 MiB (n_energies_padded = n_energies rounded up
-   to 64); larger grids are refused with CP3H_ERR_ARGUMENT.  All calls on one GPU share its
+   96 (n_rows + 1) n_energies_padded bytes <= 128 MiB (n_energies_padded = n_energies rounded up
+   to 64; each of the three parts rounded up to 256 bytes), e.g. 8192^2 on 100 shells; larger grids are refused with CP3H_ERR_ARGUMENT.  All calls on one GPU share its
    scratch: calls whose device work overlaps (different streams, no ordering) race; calls on one
    stream are ordered.  The first call on a device looks the scratch's address up (make it
    outside graph capture). */
