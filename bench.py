@@ -95,12 +95,15 @@ def time_steps(props, grids, steps, warmup, sync, after_warmup=None):
     return time.perf_counter() - t0
 
 
-def kernel_times(props, grids, reps):
+def kernel_times(props, grids, reps, preroll_s=0.25):
     """Average duration of one set-up and one pair-kernel launch over the variants, each kernel
-    launched `reps` times back to back between two HIP events (cp3h_time_kernels), after one
-    untimed round (the clocks of an idle GPU take a few launches to come up)."""
-    for p, g in zip(props, grids):
-        p.time_kernels(g.antinu, 2)
+    launched `reps` times back to back between two HIP events (cp3h_time_kernels), after at
+    least `preroll_s` seconds of untimed back-to-back launches (the clocks of a GPU that has been
+    idle or lightly loaded take tens of milliseconds of steady work to come up)."""
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < preroll_s:
+        for p, g in zip(props, grids):
+            p.time_kernels(g.antinu, 10)
     setup, pairs = [], []
     for p, g in zip(props, grids):
         su, pa = p.time_kernels(g.antinu, reps)
