@@ -197,6 +197,17 @@ __device__ inline __attribute__((always_inline)) LayerDiag layer_diag(double2 r,
     return d;
 }
 
+// The same, also adding both phases (half turns) to tsum: det D = e^{i pi (t_1 + t_2)}.
+__device__ inline __attribute__((always_inline)) LayerDiag layer_diag(double2 r, double dist, double& tsum) {
+    const double t1 = r.x * dist, t2 = r.y * dist;
+    tsum += t1;
+    tsum += t2;
+    LayerDiag d;
+    phase_factor(t1, d.c[0], d.s[0]);
+    phase_factor(t2, d.c[1], d.s[1]);
+    return d;
+}
+
 // Z <- D M Z for a real 3x3 M and D = diag(1, d_1, d_2): 3 x kCols complex Z.
 template <int kCols>
 __device__ inline __attribute__((always_inline)) void basis_step(const float m[3][3], const LayerDiag* d,
@@ -230,6 +241,41 @@ __device__ inline __attribute__((always_inline)) void basis_step(const float m[3
                 Z[a][j] = N[a][j];
             }
         }
+}
+
+// The unitary 3x3 Z from its first two columns and its determinant e^{i pi tsum}: columns 0 and
+// 1 are normalised (their norms drift from 1 by the float rounding of the running product, the
+// dominant error on long paths), and column 2 = det(Z) conj(c_0 x c_1), the cofactor identity of
+// a unitary matrix (adj Z = det(Z) Z^dagger).
+__device__ inline __attribute__((always_inline)) void complete_columns(const Cx<float> Z2[3][2], double tsum,
+                                                                       Cx<float> Z[3][3]) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        float n2 = Z2[0][j].re * Z2[0][j].re;
+        n2 = __builtin_fmaf(Z2[0][j].im, Z2[0][j].im, n2);
+#pragma unroll
+        for (int a = 1; a < 3; a++) {
+            n2 = __builtin_fmaf(Z2[a][j].re, Z2[a][j].re, n2);
+            n2 = __builtin_fmaf(Z2[a][j].im, Z2[a][j].im, n2);
+        }
+        const float inv = __builtin_amdgcn_rsqf(n2);
+#pragma unroll
+        for (int a = 0; a < 3; a++) Z[a][j] = Cx<float>{Z2[a][j].re * inv, Z2[a][j].im * inv};
+    }
+    float dc, ds;
+    phase_factor(tsum, dc, ds);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int b = (a + 1) % 3, c = (a + 2) % 3;
+        Cx<float> x = cmul(Z[b][0], Z[c][1]);
+        cmac(x, Cx<float>{-Z[c][0].re, -Z[c][0].im}, Z[b][1]);
+        // det * conj(x)
+        float re = dc * x.re, im = ds * x.re;
+        re = __builtin_fmaf(ds, x.im, re);
+        im = __builtin_fmaf(-dc, x.im, im);
+        Z[a][2] = Cx<float>{re, im};
+    }
 }
 
 // Atmosphere: G = S_0 W^dagger[:, (e, mu)] = W^dagger[:, (e, mu)] + sum_k v_k w_k z_k^T,
@@ -285,8 +331,10 @@ __device__ inline __attribute__((always_inline)) void block_probabilities(const 
 // Only the flavour rows and columns e, mu of T are needed (unitarity gives the rest).  Two forms
 // of the running product for L >= 2:
 //  * eigenbasis (kEigen, single-grid kernels): Y = O_{L-1} Z with Z = D_{L-1} M_{L-1} ... D_1 M_1,
-//    M_s = O_s^T O_{s-1} (O_0 = I) from the table: one shell costs a real 3x3 times complex 3x3
-//    product and two complex row scalings.  With Z' = M_L Z = O_L^T Y,
+//    M_s = O_s^T O_{s-1} (O_0 = I) from the table.  Z is unitary with det Z = prod_s det D_s
+//    (det M_s = 1), so only its first two columns are carried: one shell costs a real 3x3 times
+//    complex 3x2 product, two complex row scalings and two FP64 phase sums; after Z' = M_L Z the
+//    columns are normalised and the third is det(Z') conj(c_0 x c_1).  With Z' = O_L^T Y,
 //      T[x][y] = sum_c (Z' a_x)_c D_L,c (Z' g_y)_c,
 //    a_e = e_0, a_mu = row mu of W (as a column), g_y = G[:, y];
 //  * rank-1 (batched kernel, whose tables then need no basis changes): Y += v_k (w_k v_k^T Y)
@@ -367,24 +415,26 @@ __device__ inline void propagate_rows_mixed(int L, const double* const* gl, cons
     }
     if constexpr (kEigen) {
         ShellBasis b;
-        Cx<float> Z[kRows][3][3];
+        Cx<float> Z2[kRows][3][2];   // columns 0, 1 of Z
+        double tsum[kRows];          // det Z = e^{i pi tsum}
         load_basis(M, R, st, off, 1, b);
 #pragma unroll
         for (int k = 0; k < kRows; k++) {           // Z = D_1 M_1
-            const LayerDiag d = layer_diag(b.r, gl[k][1]);
+            tsum[k] = 0.0;
+            const LayerDiag d = layer_diag(b.r, gl[k][1], tsum[k]);
 #pragma unroll
-            for (int j = 0; j < 3; j++) {
-                Z[k][0][j] = Cx<float>{b.m[0][j], 0.0f};
+            for (int j = 0; j < 2; j++) {
+                Z2[k][0][j] = Cx<float>{b.m[0][j], 0.0f};
 #pragma unroll
-                for (int a = 1; a < 3; a++) Z[k][a][j] = Cx<float>{d.c[a - 1] * b.m[a][j], d.s[a - 1] * b.m[a][j]};
+                for (int a = 1; a < 3; a++) Z2[k][a][j] = Cx<float>{d.c[a - 1] * b.m[a][j], d.s[a - 1] * b.m[a][j]};
             }
         }
         auto shell = [&](int i) {
             load_basis(M, R, st, off, i, b);
 #pragma unroll
             for (int k = 0; k < kRows; k++) {
-                const LayerDiag d = layer_diag(b.r, gl[k][i]);
-                basis_step<3>(b.m, &d, Z[k]);
+                const LayerDiag d = layer_diag(b.r, gl[k][i], tsum[k]);
+                basis_step<2>(b.m, &d, Z2[k]);
             }
         };
         // one row: unrolled twice for independent work across shells; two rows carry enough
@@ -400,25 +450,27 @@ __device__ inline void propagate_rows_mixed(int L, const double* const* gl, cons
         LayerDiag dL[kRows];
 #pragma unroll
         for (int k = 0; k < kRows; k++) {
-            basis_step<3>(b.m, nullptr, Z[k]);      // Z' = M_L Z
+            basis_step<2>(b.m, nullptr, Z2[k]);     // Z' = M_L Z, columns 0, 1
             dL[k] = layer_diag(b.r, gl[k][L]);
         }
         load_table(Q, R, st, off, 0, t);
 #pragma unroll
         for (int k = 0; k < kRows; k++) {
+            Cx<float> Z[3][3];                      // Z'
+            complete_columns(Z2[k], tsum[k], Z);
             atmosphere_columns(t, gl[k][0], wmu, G[k]);
             // h_x = D_L Z' a_x for x = e, mu, and u_y = Z' g_y
             Cx<float> h[2][3], u[3][2];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                h[0][c] = Z[k][c][0];
-                h[1][c] = cmul(w1, Z[k][c][1]);
-                cmac(h[1][c], w2, Z[k][c][2]);
+                h[0][c] = Z[c][0];
+                h[1][c] = cmul(w1, Z[c][1]);
+                cmac(h[1][c], w2, Z[c][2]);
 #pragma unroll
                 for (int y = 0; y < 2; y++) {
-                    Cx<float> acc = cmul(Z[k][c][0], G[k][0][y]);
-                    cmac(acc, Z[k][c][1], G[k][1][y]);
-                    cmac(acc, Z[k][c][2], G[k][2][y]);
+                    Cx<float> acc = cmul(Z[c][0], G[k][0][y]);
+                    cmac(acc, Z[c][1], G[k][1][y]);
+                    cmac(acc, Z[c][2], G[k][2][y]);
                     u[c][y] = acc;
                 }
             }

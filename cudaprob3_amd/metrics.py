@@ -10,16 +10,19 @@ min/max and bit operations not counted:
     - L = 0: single-grid kernels: the vacuum block from the constants U_xk conj(U_yk) (64);
       batched kernel: the atmosphere columns G (76) and the block from G (28);
     - L = 1: G (76), the innermost shell on G as two rank-1 terms (112) and the block (28);
-    - L >= 2, eigenbasis form (single-grid kernels): the first shell D_1 M_1 (12), per further
-      shell of the way in one basis change and phase scaling (126), the innermost basis change
-      (90), G (76), h = D_L Z' a_x (66), u = Z' G (132) and the block (88);
+    - L >= 2, eigenbasis form (single-grid kernels, two columns carried): the first shell
+      D_1 M_1 (8), per further shell of the way in one basis change and phase scaling of two
+      columns (84) and two FP64 phase sums, the innermost basis change (60), the completion of
+      Z' (normalised columns 34, cross product 42, times the determinant 18, and the
+      determinant's phase factor 22 + 4 FP64), G (76), h = D_L Z' a_x (66), u = Z' G (132) and
+      the block (88);
     - L >= 2, rank-1 form (batched kernel): the first shell (54), per further shell two rank-1
       terms on the 3x3 running product (168), G (76), Y G (132), the innermost shell on two
       columns (112) and the block through W Y^T (130).
-  FP32 per pair: L = 0: 132 (single-grid) or 172 (batched), L = 1: 330, L >= 2: 170 L + 280
-  (eigenbasis) or 214 L + 236 (rank-1); FP64: 8 (L + 1).  This is the roofline's numerator; it
-  agrees within about 1 % with the FP32/FP64 flops that rocprofv3's
-  SQ_INSTS_VALU_{ADD,MUL,FMA}_F{32,64} counters report.
+  FP32 per pair: L = 0: 132 (single-grid) or 172 (batched), L = 1: 330, L >= 2: 128 L + 446
+  (eigenbasis) or 214 L + 236 (rank-1); FP64: 8 (L + 1), plus 2 L + 2 for L >= 2 in the
+  eigenbasis form.  This is the roofline's numerator; it agrees within a few per cent with the
+  FP32/FP64 flops that rocprofv3's SQ_INSTS_VALU_{ADD,MUL,FMA}_F{32,64} counters report.
 * The reference algorithm as written (physics.hpp:268-702), kept as a measure of the work the
   reference would do for the same results (``reference_work``): one layer step
   (get_transition_matrix) = getMfast 69 + get_product 788 + getA 874 = 1731 flops (plus 2 sqrt,
@@ -54,10 +57,12 @@ MIXED_PROBS = 22              # four |T|^2 and five unitarity closures
 MIXED_BLOCK_SHALLOW = 28      # the 2x2 block from G for L < 2
 MIXED_INNERMOST = 112         # a shell on two columns as two rank-1 terms (apply_layer<2>)
 MIXED_VACUUM_BLOCK = 64       # L = 0, single-grid: delta + sum_k w_k U_xk conj(U_yk) for four entries
-# eigenbasis form (L >= 2)
-EIGEN_FIRST_SHELL = 12        # Z = D_1 M_1
-EIGEN_SHELL = 126             # Z <- D_i M_i Z
-EIGEN_INNERMOST = 90          # Z' = M_L Z
+# eigenbasis form (L >= 2), two columns of Z carried
+EIGEN_FIRST_SHELL = 8         # Z = D_1 M_1
+EIGEN_SHELL = 84              # Z <- D_i M_i Z
+EIGEN_SHELL_F64 = 2           # the determinant's phase sum
+EIGEN_INNERMOST = 60          # Z' = M_L Z
+EIGEN_COMPLETE = 94           # normalised columns, c_0 x c_1 times the determinant
 EIGEN_TAIL = 66 + 132 + 88    # h = D_L Z' a_x, u = Z' G, the 2x2 block
 # rank-1 form (L >= 2)
 RANK1_FIRST_SHELL = 54        # init_layer
@@ -77,12 +82,14 @@ def mixed_flops_per_pair(L, eigenbasis: bool = True):
         shallow = np.where(L == 0, 2 * MIXED_PHASE_F32 + MIXED_VACUUM_BLOCK + MIXED_PROBS, shallow)
     if eigenbasis:
         deep = (2 * MIXED_PHASE_F32 + 2 * MIXED_DIAG_F32 * L + EIGEN_FIRST_SHELL + EIGEN_SHELL * (L - 2) +
-                EIGEN_INNERMOST + MIXED_ATMOSPHERE + EIGEN_TAIL + MIXED_PROBS)
+                EIGEN_INNERMOST + EIGEN_COMPLETE + MIXED_DIAG_F32 + MIXED_ATMOSPHERE + EIGEN_TAIL + MIXED_PROBS)
     else:
         deep = (2 * MIXED_PHASE_F32 * (L + 1) + RANK1_FIRST_SHELL + RANK1_SHELL * (L - 2) + MIXED_ATMOSPHERE +
                 RANK1_Y_TIMES_G + MIXED_INNERMOST + RANK1_BLOCK + MIXED_PROBS)
     f32 = np.where(L >= 2, deep, shallow)
     f64 = 2 * MIXED_PHASE_F64 * (L + 1)
+    if eigenbasis:
+        f64 = f64 + np.where(L >= 2, EIGEN_SHELL_F64 * (L - 1) + MIXED_PHASE_F64, 0)
     return f32, f64
 
 

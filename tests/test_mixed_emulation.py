@@ -250,12 +250,19 @@ def eigenbasis_tables(Hr, E, rhos, antinu):
     return tables
 
 
-def emulate_eigenbasis(g, shuffle_m=False):
-    """The single-grid mixed kernel: for rows with L >= 2 the running product carried as
-    Z = D_i M_i ... D_1 M_1 (M real 3x3 basis changes, D_i = diag(1, e^{i dphi_1}, e^{i dphi_2})),
-    then Z' = M_L Z and T[x][y] = sum_c (Z' a_x)_c D_L,c (Z' G)_c,y with a_e = e_0, a_mu = row mu
-    of W; for L = 0 the vacuum form T = I + sum_{k=1,2} w_k U_xk conj(U_yk); L = 1 as in
-    emulate()."""
+def cross3(a, b):
+    """Row-wise cross product of [n, 3] arrays."""
+    return np.stack([a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1], a[:, 2] * b[:, 0] - a[:, 0] * b[:, 2],
+                     a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]], axis=1)
+
+
+def emulate_eigenbasis(g, shuffle_m=False, det_phase=True):
+    """The single-grid mixed kernel: for rows with L >= 2 the first two columns of the running
+    product Z = D_i M_i ... D_1 M_1 (M real 3x3 basis changes, D_i = diag(1, e^{i dphi_1},
+    e^{i dphi_2})) and the FP64 sum of the phases (half turns), then the first two columns of
+    Z' = M_L Z, normalised, completed by det(Z') conj(c_0 x c_1) with det = e^{i pi sum}, and
+    T[x][y] = sum_c (Z' a_x)_c D_L,c (Z' G)_c,y with a_e = e_0, a_mu = row mu of W; for L = 0 the
+    vacuum form T = I + sum_{k=1,2} w_k U_xk conj(U_yk); L = 1 as in emulate()."""
     p = Propagator(len(g["cosines"]), len(g["energies"]))
     p.setEnergyList(g["energies"])
     p.setCosineList(g["cosines"])
@@ -300,16 +307,22 @@ def emulate_eigenbasis(g, shuffle_m=False):
             out[ic] = base[ic]
             continue
 
-        def d(i):
+        tsum = np.zeros(n_e)
+
+        def d(i, add=True):
+            nonlocal tsum
             rate = tables[i][0]
-            return [phase_factor(rate[0] * dist[i]), phase_factor(rate[1] * dist[i])]
+            t1, t2 = rate[0] * dist[i], rate[1] * dist[i]
+            if add:
+                tsum = tsum + t1 + t2
+            return [phase_factor(t1), phase_factor(t2)]
 
         def Mof(i):
             M = tables[i][2]
             return M[:, ::-1, :] if shuffle_m else M
 
         d1 = d(1)
-        Z = Mof(1).astype(np.complex64)
+        Z = Mof(1)[:, :, :2].astype(np.complex64)
         Z[:, 1, :] *= d1[0][:, None]
         Z[:, 2, :] *= d1[1][:, None]
         for i in range(2, L):
@@ -318,7 +331,11 @@ def emulate_eigenbasis(g, shuffle_m=False):
             Z[:, 1, :] *= di[0][:, None]
             Z[:, 2, :] *= di[1][:, None]
         Zp = (Mof(L) @ Z).astype(np.complex64)
-        dL = d(L)
+        Zp = (Zp / np.sqrt((np.abs(Zp) ** 2).sum(1, keepdims=True)).astype(F32)).astype(np.complex64)
+        det = phase_factor(tsum) if det_phase else np.ones(n_e, np.complex64)
+        c2 = (det[:, None] * np.conj(cross3(Zp[:, :, 0], Zp[:, :, 1]))).astype(np.complex64)
+        Zp = np.concatenate([Zp, c2[:, :, None]], axis=2)
+        dL = d(L, add=False)
         G = apply_layer(np.repeat(Wc[None], n_e, axis=0), tables[0][1], [phase_factor_m1(tables[0][0][k] * dist[0])
                                                                            for k in (0, 1)])
         Ug = (Zp @ G).astype(np.complex64)                                   # [e][c][y]
@@ -354,3 +371,30 @@ def test_eigenbasis_emulation_detects_a_wrong_basis_change():
     g = load_golden("prem59_nu")
     bad = emulate_eigenbasis(g, shuffle_m=True)
     assert np.abs(bad - g["P"]).max() > 1e-2
+
+
+def test_eigenbasis_emulation_detects_a_missing_determinant():
+    """The third column of Z' is det(Z') conj(c_0 x c_1): leaving out the determinant's phase
+    breaks the bar."""
+    g = load_golden("prem59_nu")
+    bad = emulate_eigenbasis(g, det_phase=False)
+    assert np.abs(bad - g["P"]).max() > 1e-2
+
+
+def test_eigenbasis_long_paths_on_a_100_shell_earth():
+    """Paths of up to 199 layers (100 equal shells): the completed two-column product stays within
+    the bar against the oracle (the normalisation of the columns removes the norm drift that
+    dominates over so many products)."""
+    from cudaprob3_amd import workloads as W
+    radii, rhos = W.fine100_model()
+    cos = np.linspace(-1.0, -0.3, 12)
+    E = np.geomspace(0.1, 1000.0, 48)
+    for antinu, dm32 in ((0, 2.5e-3), (1, -2.5e-3)):
+        params = list(W.NUFIT)
+        params[5] = dm32
+        g = dict(cosines=cos, energies=E, radii=radii[::-1].copy(), rhos=rhos[::-1].copy(), h_km=22.0,
+                 params=np.array(params), antinu=antinu)
+        g["P"] = oracle.propagate(antinu, cos, E, g["radii"], g["rhos"], 22.0, params)
+        err = np.abs(emulate_eigenbasis(g) - g["P"]).max()
+        print(f"fine100 antinu={antinu} dm32={dm32}: emulated max|dP| {err:.2e}")
+        assert err <= 0.5 * MIXED_TOL
