@@ -327,7 +327,9 @@ def run_scan(args, sync, steps, warmup, cpu=True):
         ref += M.grid_flops(h.maxlayers, h.n_energies)
     out_bytes = 36 * b.n_pairs
     roof = {"bound": "valu", **M.valu_roofline(f32, f64, k_ms),
-            "traffic": traffic_from_profile("scan1024", "mixed"), "kernel": f"batch_pairs<{waves}, false>",
+            "traffic": traffic_from_profile("scan1024", "mixed"),
+            "counter_flops_per_launch": traffic_from_profile("scan1024", "mixed", "counter_flops_per_launch"),
+            "kernel": f"batch_pairs<{waves}, false>",
             "kernel_ms": k_ms, "setup_kernel_ms": setup_ms, "launch_reps": args.kernel_reps_batch,
             "reference_work_TFLOPs": ref / (k_ms * 1e-3) / 1e12, "output_GBps": out_bytes / (k_ms * 1e-3) / 1e9}
     result = {
@@ -393,7 +395,9 @@ def run_mcmc(args, sync, steps, warmup, binned_steps, cpu=True):
         "latency_ms": {"p50": p50, "p99": p99, "mean": float(np.mean(st.step_ms))},
         "acceptance": st.accepted / max(st.steps, 1),
         "roofline": {"bound": "valu", **M.valu_roofline(f32, f64, k_ms),
-                     "traffic": traffic_from_profile("mcmc8192", "mixed"), "kernel": "pairs_mixed_kernel<true>",
+                     "traffic": traffic_from_profile("mcmc8192", "mixed"),
+                     "counter_flops_per_launch": traffic_from_profile("mcmc8192", "mixed", "counter_flops_per_launch"),
+                     "kernel": "pairs_mixed_kernel<true>",
                      "kernel_ms": k_ms, "setup_kernel_ms": setup_ms, "launch_reps": args.kernel_reps_batch,
                      "reference_work_TFLOPs": ref / (k_ms * 1e-3) / 1e12},
     }

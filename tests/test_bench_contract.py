@@ -44,3 +44,47 @@ def test_jitter12_grid_flops():
     ml = M.maxlayers_of(g.cosines, g.radii, g.rhos)
     assert ml.max() == 12 and (ml == 0).sum() == 1024
     assert abs(M.grid_flops(ml, len(g.energies)) / 1e9 - 34.60) < 0.005
+
+
+def _profile(name):
+    import json
+    with open(os.path.join(ROOT, "profiles", name)) as f:
+        return json.load(f)
+
+
+def test_mixed_flops_agree_with_the_committed_counters():
+    """The roofline's numerator for the single-grid mixed kernels (eigenbasis form, vacuum rows,
+    the likelihood's FP64 sums) against the flops rocprofv3's SQ_INSTS_VALU_{ADD,MUL,FMA}_F{32,64}
+    counters read per launch (profiles/traffic_*_mixed.json): within 3 %.  On the 12-shell grid a
+    count without the completion of Z' (normalisation, cross product, determinant; about 10 % of
+    a deep pair's flops there) would miss by more than that."""
+    g = W.jitter12_2048()
+    ml = M.maxlayers_of(g.cosines, g.radii, g.rhos)
+    f32, f64 = M.mixed_grid_flops(ml, len(g.energies))
+    counted = _profile("traffic_jitter12_2048_mixed.json")["counter_flops_per_launch"]
+    assert abs((f32 + f64) / counted - 1) < 0.03
+    g = W.fine100_grid(8192)
+    ml = M.maxlayers_of(g.cosines, g.radii, g.rhos)
+    f32, f64 = M.mixed_grid_flops(ml, len(g.energies), likelihood=True)
+    counted = _profile("traffic_mcmc8192_mixed.json")["counter_flops_per_launch"]
+    assert abs((f32 + f64) / counted - 1) < 0.03
+    # the batched kernel's rank-1 form does not describe this kernel
+    f32r, f64r = M.mixed_grid_flops(ml, len(g.energies), likelihood=True, eigenbasis=False)
+    assert abs((f32r + f64r) / counted - 1) > 0.1
+
+
+def test_batched_flops_agree_with_the_committed_counters():
+    """The batched kernel (rank-1 form) over the 1024 scan sets: the counters also count the lanes
+    of each set's last 64-energy tile beyond its energies (masked lanes still issue), so the
+    algorithmic count with energies padded to 64 must match them within 3 %; the roofline uses the
+    unpadded count (useful work only), about 5 % lower."""
+    useful = padded = 0.0
+    for s in W.scan_sets():
+        ml = M.maxlayers_of(s.cosines, s.radii, s.rhos)
+        a, b = M.mixed_grid_flops(ml, len(s.energies), eigenbasis=False)
+        useful += a + b
+        a, b = M.mixed_grid_flops(ml, -(-len(s.energies) // 64) * 64, eigenbasis=False)
+        padded += a + b
+    counted = _profile("traffic_scan1024_mixed.json")["counter_flops_per_launch"]
+    assert abs(padded / counted - 1) < 0.03
+    assert useful < padded
