@@ -197,49 +197,77 @@ __device__ inline void layer_amplitude(const OscParams& p, const double* __restr
         }
 }
 
-// B <- A B in place, one column at a time (column j of the product needs column j of B only),
-// with the summation order of math.hpp:50-64.
-__device__ inline void mul_left(const Cd A[3][3], Cd B[3][3]) {
+// The two running products of every lane of a workgroup, parked in LDS while a layer's
+// amplitude is formed: element e of lane t at park[e][t] (fin in e = 0..17, mantle in 18..35,
+// (row * 3 + column) * 2 + {re, im}).  Each lane touches only its own column, so no barrier is
+// needed.  With them in registers the kernel needed 316 registers per lane (one wavefront per
+// SIMD); parked, the amplitude's own working set fits 256 (two wavefronts per SIMD).
+constexpr int kPlainBlock = 128;                 // 2 wavefronts per workgroup
+constexpr int kPlainRows = kPlainBlock / kWave;  // cosine rows per workgroup
+constexpr int kFin = 0, kMantle = 18;
+
+struct Parked {
+    double (*p)[kPlainBlock];
+    int t;
+    __device__ Cd get(int base, int a, int b) const {
+        const int e = base + (a * 3 + b) * 2;
+        return Cd{p[e][t], p[e + 1][t]};
+    }
+    __device__ void put(int base, int a, int b, Cd v) const {
+        const int e = base + (a * 3 + b) * 2;
+        p[e][t] = v.re;
+        p[e + 1][t] = v.im;
+    }
+};
+
+// B <- A B for the parked B, one column at a time (column j of the product needs column j of B
+// only), with the summation order of math.hpp:50-64.
+__device__ inline void mul_left(const Cd A[3][3], const Parked& P, int base) {
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        Cd col[3];
+        Cd b[3], col[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) b[k] = P.get(base, k, j);
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             Cd c{0.0, 0.0};
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                c.re += A[i][k].re * B[k][j].re - A[i][k].im * B[k][j].im;
-                c.im += A[i][k].im * B[k][j].re + A[i][k].re * B[k][j].im;
+                c.re += A[i][k].re * b[k].re - A[i][k].im * b[k].im;
+                c.im += A[i][k].im * b[k].re + A[i][k].re * b[k].im;
             }
             col[i] = c;
         }
 #pragma unroll
-        for (int i = 0; i < 3; i++) B[i][j] = col[i];
+        for (int i = 0; i < 3; i++) P.put(base, i, j, col[i]);
     }
 }
 
-// A <- A B in place, one row at a time, same summation order.
-__device__ inline void mul_right(Cd A[3][3], const Cd B[3][3]) {
+// A <- A B for the parked A, one row at a time, same summation order.
+__device__ inline void mul_right(const Parked& P, int base, const Cd B[3][3]) {
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        Cd row[3];
+        Cd a[3], row[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) a[k] = P.get(base, i, k);
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             Cd c{0.0, 0.0};
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                c.re += A[i][k].re * B[k][j].re - A[i][k].im * B[k][j].im;
-                c.im += A[i][k].im * B[k][j].re + A[i][k].re * B[k][j].im;
+                c.re += a[k].re * B[k][j].re - a[k].im * B[k][j].im;
+                c.im += a[k].im * B[k][j].re + a[k].re * B[k][j].im;
             }
             row[j] = c;
         }
 #pragma unroll
-        for (int j = 0; j < 3; j++) A[i][j] = row[j];
+        for (int j = 0; j < 3; j++) P.put(base, i, j, row[j]);
     }
 }
 
-__global__ __launch_bounds__(kBlock) void propagate_plain_f64(GridDesc g, OscParams p, double* __restrict__ out) {
+__global__ __launch_bounds__(kPlainBlock) void propagate_plain_f64(GridDesc g, OscParams p, double* __restrict__ out) {
     __shared__ double ax[324];
+    __shared__ double park[36][kPlainBlock];
     for (int t = threadIdx.x; t < 81; t += blockDim.x) {
         const int n = t / 27, m = (t / 9) % 3, i = (t / 3) % 3, j = t % 3;
         axfac(p, n, m, i, j, ax + 4 * t);
@@ -248,9 +276,9 @@ __global__ __launch_bounds__(kBlock) void propagate_plain_f64(GridDesc g, OscPar
 
     const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
     const int tiles = (g.n_e + kWave - 1) / kWave;
-    const int row_groups = (g.n_c + 3) / 4;
+    const int row_groups = (g.n_c + kPlainRows - 1) / kPlainRows;
     const int tile = lb / row_groups;
-    const int ic = (lb % row_groups) * 4 + threadIdx.x / kWave;
+    const int ic = (lb % row_groups) * kPlainRows + threadIdx.x / kWave;
     if (tile >= tiles || ic >= g.n_c) return;
     const int ie = tile * kWave + threadIdx.x % kWave;
     if (ie >= g.n_e) return;
@@ -262,18 +290,19 @@ __global__ __launch_bounds__(kBlock) void propagate_plain_f64(GridDesc g, OscPar
     const double earth_path = -2.0 * cz * kREarthCm;
 
     // physics.hpp:645-677: fin = T_i fin for every layer, mantle = mantle T_i for 1 <= i < L
-    Cd T[3][3], fin[3][3], mantle[3][3];
+    const Parked P{park, int(threadIdx.x)};
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) mantle[i][j] = Cd{i == j ? 1.0 : 0.0, 0.0};
+        for (int j = 0; j < 3; j++) P.put(kMantle, i, j, Cd{i == j ? 1.0 : 0.0, 0.0});
 
+    Cd T[3][3];
     for (int i = 0; i <= L; i++) {
         const double dist = layer_distance(g.radii, i, L, path, earth_path, cz);
         const double dens = layer_density(g.rhos, i, L);
         // the AXFAC table is loop-invariant: hoisted out of the layer loop, its 324 doubles
-        // would stay live in registers across the loop and spill (1.7 KB of scratch per lane); an
-        // offset the compiler cannot see through keeps its LDS reads inside each layer step
+        // would stay live in registers across the loop and spill; an offset the compiler cannot
+        // see through keeps its LDS reads inside each layer step
         int opaque_zero = 0;
         asm volatile("" : "+s"(opaque_zero));
         layer_amplitude(p, ax + opaque_zero, E, dens * kYe, dist / kKm2Cm, T);
@@ -281,10 +310,10 @@ __global__ __launch_bounds__(kBlock) void propagate_plain_f64(GridDesc g, OscPar
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
-                for (int b = 0; b < 3; b++) fin[a][b] = T[a][b];
+                for (int b = 0; b < 3; b++) P.put(kFin, a, b, T[a][b]);
         } else {
-            mul_left(T, fin);
-            if (i < L) mul_right(mantle, T);
+            mul_left(T, P, kFin);
+            if (i < L) mul_right(P, kMantle, T);
         }
     }
 
@@ -292,17 +321,22 @@ __global__ __launch_bounds__(kBlock) void propagate_plain_f64(GridDesc g, OscPar
     const std::size_t plane = std::size_t(g.n_c) * g.n_e;
     double* o = out + std::size_t(ic) * g.n_e + ie;
 #pragma unroll
-    for (int a = 0; a < 3; a++)
+    for (int a = 0; a < 3; a++) {
+        Cd f[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) f[k] = P.get(kFin, k, a);
 #pragma unroll
         for (int b = 0; b < 3; b++) {
             Cd v{0.0, 0.0};
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                v.re += mantle[b][k].re * fin[k][a].re - mantle[b][k].im * fin[k][a].im;
-                v.im += mantle[b][k].im * fin[k][a].re + mantle[b][k].re * fin[k][a].im;
+                const Cd m = P.get(kMantle, b, k);
+                v.re += m.re * f[k].re - m.im * f[k].im;
+                v.im += m.im * f[k].re + m.re * f[k].im;
             }
             o[(a * 3 + b) * plane] = v.re * v.re + v.im * v.im;
         }
+    }
 }
 
 // Length in km of layer i of cosine row ic (0 beyond the row's crossed layers).
@@ -572,8 +606,8 @@ __global__ __launch_bounds__(kBlock) void batch_setup(BatchDesc b) {
 hipError_t launch_plain_f64(const GridDesc& g, const OscParams& p, double* out, hipStream_t s) {
     if (g.n_c == 0 || g.n_e == 0) return hipSuccess;
     const unsigned tiles = (g.n_e + kWave - 1) / kWave;
-    const unsigned blocks = round_up(tiles * ((g.n_c + 3) / 4), kXcds);
-    hipLaunchKernelGGL(propagate_plain_f64, dim3(blocks), dim3(kBlock), 0, s, g, p, out);
+    const unsigned blocks = round_up(tiles * ((g.n_c + kPlainRows - 1) / kPlainRows), kXcds);
+    hipLaunchKernelGGL(propagate_plain_f64, dim3(blocks), dim3(kPlainBlock), 0, s, g, p, out);
     return hipGetLastError();
 }
 
