@@ -189,14 +189,16 @@ def run_jitter12(args, sync):
     workload = f"jitter12_{args.n}"
     pairs_per_step = sum(g.n_pairs for g in grids)
     log(f"[bench] workload {workload}: {len(grids)} variants x {grids[0].n_pairs} pairs, precision {args.precision}")
-    # the four propagators share one stream (default priority), so their kernels run one after
-    # the other and every kernel duration is its own: with one stream per propagator the steps
-    # are 5 % shorter, but the pair kernels of different variants overlap and their durations
-    # (HIP events, rocprof) then include each other's work
+    # each propagator enqueues on its own stream (created by the handle, default priority), as
+    # each of the reference's CudaPropagatorSingle objects does, so one variant's set-up kernel
+    # can run beside the previous variant's pair kernel.  The kernel durations of the roofline
+    # are timed separately, one propagator at a time (kernel_times), where nothing overlaps.
+    # --shared-stream puts all four on one stream instead.
     props = [make_prop(g, args.precision) for g in grids]
-    shared = torch.cuda.Stream()
-    for p in props:
-        p.set_stream(shared.cuda_stream)
+    shared = torch.cuda.Stream() if args.shared_stream else None
+    if shared is not None:
+        for p in props:
+            p.set_stream(shared.cuda_stream)
 
     # kernel durations between the warm-up and the timed steps: back-to-back launches, which also
     # keep the clocks up for the timed steps
@@ -252,7 +254,8 @@ def run_jitter12(args, sync):
         "data": "synthetic",
         "config": {"workload": f"{workload}: {args.n} cosZ in [-1,1] x {args.n} E log 0.05-1000 GeV, "
                                "12 jittered PREM shells, h=22 km, nu/nubar x NO/IO (4 grids per step)",
-                   "precision": args.precision, "pairs_per_step": pairs_per_step},
+                   "precision": args.precision, "pairs_per_step": pairs_per_step,
+                   "streams": "one shared" if args.shared_stream else "one per propagator"},
         "roofline": roof,
         "d2h": {"ms_per_grid": d2h_ms, "bytes_per_grid": d2h_bytes, "GBps": d2h_bytes / (d2h_ms * 1e-3) / 1e9,
                 "pairs_per_s_including_d2h": grids[0].n_pairs / ((dt / args.steps / len(grids)) + d2h_ms * 1e-3),
@@ -645,6 +648,8 @@ def main():
                     help="back-to-back launches for the kernel durations of the scan1024 and mcmc8192 lines")
     ap.add_argument("--sub-steps", type=int, default=20, help="timed steps of the scan1024 / mcmc8192 sub-objects")
     ap.add_argument("--sub-warmup", type=int, default=3)
+    ap.add_argument("--shared-stream", action="store_true",
+                    help="jitter12: run the four propagators on one stream (default: each on its own)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other precisions' throughput")
