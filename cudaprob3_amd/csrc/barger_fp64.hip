@@ -68,6 +68,72 @@ __device__ inline void matter_masses(const OscParams& p, double E, double rho, d
         }
 }
 
+// sin x and cos x for 0 <= x <= pi/3 in FP64: their Taylor polynomials to degrees 17 and 18 in
+// Horner form (truncation below 2e-17 on that range; about one ulp in all).
+__device__ inline void sincos_third(double x, double& sn, double& cs) {
+    const double x2 = x * x;
+    double ps = 2.8114572543455206e-15;
+    ps = ps * x2 + -7.647163731819816e-13;
+    ps = ps * x2 + 1.6059043836821613e-10;
+    ps = ps * x2 + -2.505210838544172e-08;
+    ps = ps * x2 + 2.7557319223985893e-06;
+    ps = ps * x2 + -0.0001984126984126984;
+    ps = ps * x2 + 0.008333333333333333;
+    ps = ps * x2 + -0.16666666666666666;
+    sn = x + x * (x2 * ps);
+    double pc = -1.5619206968586225e-16;
+    pc = pc * x2 + 4.779477332387385e-14;
+    pc = pc * x2 + -1.1470745597729725e-11;
+    pc = pc * x2 + 2.08767569878681e-09;
+    pc = pc * x2 + -2.755731922398589e-07;
+    pc = pc * x2 + 2.48015873015873e-05;
+    pc = pc * x2 + -0.001388888888888889;
+    pc = pc * x2 + 0.041666666666666664;
+    pc = pc * x2 + -0.5;
+    cs = 1.0 + x2 * pc;
+}
+
+// The same ordered matter eigenvalues for the mixed kernels' tables, with a shorter dependency
+// chain (the set-up kernels are latency-bound): the three roots' cosines cos(th0 -+ 2 pi / 3) by
+// angle addition from one sine and cosine of th0 in [0, pi/3] (sincos_third) instead of three
+// library cosines, and no division where a reciprocal square root does.  Equal to
+// matter_masses to a few ulps of the largest eigenvalue; not used by the FP64 kernels, whose
+// bar follows the reference's own rounding.
+__device__ inline void matter_masses_fast(const OscParams& p, double E, double rho, double mm[3][3],
+                                          double mv[3][3]) {
+    const double* D = p.DM;
+    const Cd u0 = Uof(p, 0, 0), u1 = Uof(p, 0, 1), u2 = Uof(p, 0, 2);
+    const double fac = p.antinu ? kGF * E * rho : -kGF * E * rho;
+    const double alpha = fac + D[1] + D[2];
+    const double beta = D[1] * D[2] +
+        fac * (D[1] * (1.0 - u1.re * u1.re - u1.im * u1.im) + D[2] * (1.0 - u2.re * u2.re - u2.im * u2.im));
+    const double gamma = fac * D[1] * D[2] * (u0.re * u0.re + u0.im * u0.im);
+    const double disc = alpha * alpha - 3.0 * beta;
+    const double tmp = disc < 0 ? 0.0 : disc;
+    const double rs = rsqrt(tmp);                        // 1 / sqrt(tmp)
+    const double argtmp = (2.0 * alpha * alpha * alpha - 9.0 * alpha * beta + 27.0 * gamma) * (0.5 * rs * rs * rs);
+    const double arg = fmin(1.0, fmax(-1.0, argtmp));
+    double s0, c0;
+    sincos_third(acos(arg) / 3.0, s0, c0);
+    const double scale = -(2.0 / 3.0) * sqrt(tmp);      // off the acos chain
+    const double shift = D[0] - alpha / 3.0;
+    const double half_c = -0.5 * c0, rt3_s = 0.8660254037844386 * s0;
+    const double root[3] = {scale * c0 + shift, scale * (half_c + rt3_s) + shift, scale * (half_c - rt3_s) + shift};
+    double m[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int o = p.order[i];
+        m[i] = o == 0 ? root[0] : (o == 1 ? root[1] : root[2]);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            mm[i][j] = m[i] - m[j];
+            mv[i][j] = m[i] - D[3 * j];
+        }
+}
+
 // Entry (n,m) of 2EH - M_j (physics.hpp:361-379): a Hermitian matrix whose off-diagonal
 // part does not depend on j.
 struct TwoEH {
@@ -442,7 +508,7 @@ __global__ __launch_bounds__(kBlock) void shell_spectrum_f64(GridDesc g, OscPara
 // dphi_k / L in half turns per km as Rm[(s*ne_pad + e)*2 + k - 1], double.
 __device__ inline void real_eigenvectors_12(const OscParams& p, double E, double rho, double v[2][3],
                                             double mm[3][3], double mv[3][3]) {
-    matter_masses(p, E, rho, mm, mv);
+    matter_masses_fast(p, E, rho, mm, mv);
     const double fac = p.antinu ? kGF * E * rho : -kGF * E * rho;
     double H[3][3];
     H[0][0] = p.Hr[0] - fac; H[1][1] = p.Hr[1]; H[2][2] = p.Hr[2];
@@ -475,7 +541,7 @@ __device__ inline void real_eigenvectors_12(const OscParams& p, double E, double
         const double d01 = j1 ? Q[1][1] : Q[0][0];
         const bool j2 = Q[2][2] > d01;
         const double djj = j2 ? Q[2][2] : d01;
-        const double s = djj > 0.0 ? 1.0 / sqrt(djj) : 0.0;
+        const double s = djj > 0.0 ? rsqrt(djj) : 0.0;
 #pragma unroll
         for (int n = 0; n < 3; n++) v[kk][n] = (j2 ? Q[n][2] : (j1 ? Q[n][1] : Q[n][0])) * s;
     }
