@@ -325,7 +325,7 @@ def run_scan(args, sync, steps, warmup, cpu=True):
         f"set-up {setup_ms:.3f} ms per launch")
     f32 = f64 = ref = 0.0
     for h in b.hosts:
-        a, c = M.mixed_grid_flops(h.maxlayers, h.n_energies, eigenbasis=False)
+        a, c = M.mixed_grid_flops(h.maxlayers, h.n_energies, form="local")
         f32, f64 = f32 + a, f64 + c
         ref += M.grid_flops(h.maxlayers, h.n_energies)
     out_bytes = 36 * b.n_pairs

@@ -107,7 +107,7 @@ __device__ inline void pair_item(const BatchDesc& b, int k, long long local, int
     if (L0 == L1) {
         const double* gl[2] = {gbase + std::size_t(ic0) * gstride, gbase + std::size_t(ic0 + 1) * gstride};
         float p[2][3][3];
-        propagate_rows_mixed<2, false>(L0, gl, Q, nullptr, R, set.ne_pad, ie, wmu, nullptr, p);
+        propagate_rows_mixed<2, kFormEigenLocal>(L0, gl, Q, nullptr, R, set.ne_pad, ie, wmu, nullptr, p);
         store_row<kBinned>(b, set, ic0, ne0, ie, lane, p[0]);
         store_row<kBinned>(b, set, ic0 + 1, ne1, ie, lane, p[1]);
         return;
@@ -118,7 +118,7 @@ __device__ inline void pair_item(const BatchDesc& b, int k, long long local, int
         const int ic = ic0 + r;
         const double* gl[1] = {gbase + std::size_t(ic) * gstride};
         float p[1][3][3];
-        propagate_rows_mixed<1, false>(r == 0 ? L0 : L1, gl, Q, nullptr, R, set.ne_pad, ie, wmu, nullptr, p);
+        propagate_rows_mixed<1, kFormEigenLocal>(r == 0 ? L0 : L1, gl, Q, nullptr, R, set.ne_pad, ie, wmu, nullptr, p);
         store_row<kBinned>(b, set, ic, r == 0 ? ne0 : ne1, ie, lane, p[0]);
     }
 }

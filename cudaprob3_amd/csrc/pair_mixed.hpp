@@ -15,6 +15,12 @@ namespace cp3 {
 
 template <typename T> struct Cx { T re, im; };
 
+// Forms of the running product (propagate_rows_mixed): the eigenbasis form with the basis
+// changes read from the table (and the vacuum form for paths crossing no shell), or formed from
+// the table's eigenvectors.
+constexpr int kFormEigenTable = 1;
+constexpr int kFormEigenLocal = 2;
+
 __device__ inline float clamp01(float x) { return fminf(1.0f, fmaxf(0.0f, x)); }
 
 // acc += a * b for complex a, b as four fused multiply-adds (written out so the compiler
@@ -150,23 +156,6 @@ __device__ inline __attribute__((always_inline)) void apply_layer(const LayerTab
     }
 }
 
-// Y = S (the first shell): Y_ab = delta_ab + sum_k w_k v_ka v_kb, complex symmetric.
-__device__ inline __attribute__((always_inline)) void init_layer(const LayerTable& t, const LayerPhases& w,
-                                                                 Cx<float> Y[3][3]) {
-#pragma clang fp contract(off)
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int b = a; b < 3; b++) {
-            const float p0 = t.v[0][a] * t.v[0][b], p1 = t.v[1][a] * t.v[1][b];
-            Cx<float> y;
-            y.re = __builtin_fmaf(w.re[1], p1, __builtin_fmaf(w.re[0], p0, a == b ? 1.0f : 0.0f));
-            y.im = __builtin_fmaf(w.im[1], p1, w.im[0] * p0);
-            Y[a][b] = y;
-            Y[b][a] = y;
-        }
-}
-
 // One shell's basis change M_s = O_s^T O_{s-1} (rows) and phase rates at this lane's energy.
 struct ShellBasis {
     float m[3][3];
@@ -183,6 +172,43 @@ __device__ inline __attribute__((always_inline)) void load_basis(const float4* _
         b.m[r][0] = a.x; b.m[r][1] = a.y; b.m[r][2] = a.z;
     }
     b.r = *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(R + std::size_t(i) * st) + off);
+}
+
+// The basis change formed from the stored eigenvectors instead (batched kernel, whose tables hold
+// no basis changes): O_s = [v_1 x v_2, v_1, v_2] in float, M_s = O_s^T O_prev with O_prev the
+// previous shell's eigenbasis (o[c] = column c), then O_prev <- O_s.  For s = 1, O_0 = I and
+// M_1 = O_1^T.
+__device__ inline __attribute__((always_inline)) void local_basis(const LayerTable& t, bool first, float oprev[3][3],
+                                                                 ShellBasis& b) {
+#pragma clang fp contract(off)
+    const float* v1 = t.v[0];
+    const float* v2 = t.v[1];
+    float o[3][3];
+    o[0][0] = __builtin_fmaf(v1[1], v2[2], -(v1[2] * v2[1]));
+    o[0][1] = __builtin_fmaf(v1[2], v2[0], -(v1[0] * v2[2]));
+    o[0][2] = __builtin_fmaf(v1[0], v2[1], -(v1[1] * v2[0]));
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        o[1][c] = v1[c];
+        o[2][c] = v2[c];
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            if (first) {
+                b.m[r][c] = o[r][c];
+            } else {
+                float acc = o[r][0] * oprev[c][0];
+                acc = __builtin_fmaf(o[r][1], oprev[c][1], acc);
+                b.m[r][c] = __builtin_fmaf(o[r][2], oprev[c][2], acc);
+            }
+        }
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) oprev[r][c] = o[r][c];
+    b.r = t.r;
 }
 
 // d_k = e^{i dphi_k} (k = 1, 2) of a layer of length dist km
@@ -328,18 +354,17 @@ __device__ inline __attribute__((always_inline)) void block_probabilities(const 
 // phase dropped: it cancels in |T|^2), so with Y = S_{L-1} ... S_1 (the way in) the way out
 // S_1 ... S_{L-1} is Y^T, and
 //   T = W Y^T S_L Y G,   G = S_0 W^dagger[:, (e, mu)].
-// Only the flavour rows and columns e, mu of T are needed (unitarity gives the rest).  Two forms
-// of the running product for L >= 2:
-//  * eigenbasis (kEigen, single-grid kernels): Y = O_{L-1} Z with Z = D_{L-1} M_{L-1} ... D_1 M_1,
-//    M_s = O_s^T O_{s-1} (O_0 = I) from the table.  Z is unitary with det Z = prod_s det D_s
-//    (det M_s = 1), so only its first two columns are carried: one shell costs a real 3x3 times
-//    complex 3x2 product, two complex row scalings and two FP64 phase sums; after Z' = M_L Z the
-//    columns are normalised and the third is det(Z') conj(c_0 x c_1).  With Z' = O_L^T Y,
-//      T[x][y] = sum_c (Z' a_x)_c D_L,c (Z' g_y)_c,
-//    a_e = e_0, a_mu = row mu of W (as a column), g_y = G[:, y];
-//  * rank-1 (batched kernel, whose tables then need no basis changes): Y += v_k (w_k v_k^T Y)
-//    for k = 1, 2 per shell (w_k = e^{i dphi_k} - 1, sum_k v_k v_k^T = I), then
-//    T = (W Y^T)[e mu, :] S_L (Y G).
+// Only the flavour rows and columns e, mu of T are needed (unitarity gives the rest).  For L >= 2
+// the running product is carried in the shells' eigenbases (kFormEigenTable, single-grid
+// kernels; kFormEigenLocal, batched kernel):
+//   Y = O_{L-1} Z with Z = D_{L-1} M_{L-1} ... D_1 M_1, M_s = O_s^T O_{s-1} (O_0 = I), read from
+//   the table's basis changes or formed from its eigenvectors (local_basis).  Z is unitary with
+//   det Z = prod_s det D_s (det M_s = 1), so only its first two columns are carried: one shell
+//   costs a real 3x3 times complex 3x2 product, two complex row scalings and two FP64 phase sums;
+//   after Z' = M_L Z the columns are normalised and the third is det(Z') conj(c_0 x c_1).  With
+//   Z' = O_L^T Y,
+//     T[x][y] = sum_c (Z' a_x)_c D_L,c (Z' g_y)_c,
+//   a_e = e_0, a_mu = row mu of W (as a column), g_y = G[:, y].
 // L <= 1: T = W S_1 G (S_1 as two rank-1 updates of G) or W G; in the single-grid kernel a path
 // crossing no shell takes the vacuum form T = U D_0 U^dagger with host constants U_xk conj(U_yk).
 //
@@ -348,7 +373,7 @@ __device__ inline __attribute__((always_inline)) void block_probabilities(const 
 // layer lengths, hence the phases, differ), which halves the table reads per pair for kRows = 2.
 // wmu = row mu of W (OscParams::Wmu); row e of W is e_0^T.  L is wave-uniform, so every branch
 // here is a scalar one.
-template <int kRows, bool kEigen>
+template <int kRows, int kForm>
 __device__ inline void propagate_rows_mixed(int L, const double* const* gl, const float4* __restrict__ Q,
                                             const float4* __restrict__ M, const double2* __restrict__ R,
                                             std::size_t st, int ie, float4 wmu, const VacuumBlock* vac,
@@ -359,7 +384,7 @@ __device__ inline void propagate_rows_mixed(int L, const double* const* gl, cons
     const Cx<float> w1{wmu.x, wmu.y}, w2{wmu.z, wmu.w};
     LayerTable t;
     Cx<float> G[kRows][3][2];   // columns e, mu of S_0 W^dagger
-    if (kEigen && L == 0) {
+    if (kForm == kFormEigenTable && L == 0) {
         // one vacuum layer: T[x][y] = delta_xy + sum_{k=1,2} w_k U_xk conj(U_yk), w_k = e^{i dphi_k} - 1
         // (W S_0 W^dagger = U D U^dagger in vacuum); only the phase rates are read from the table
         const double2 r = *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(R) + off);
@@ -413,11 +438,25 @@ __device__ inline void propagate_rows_mixed(int L, const double* const* gl, cons
         }
         return;
     }
-    if constexpr (kEigen) {
+    {
         ShellBasis b;
         Cx<float> Z2[kRows][3][2];   // columns 0, 1 of Z
         double tsum[kRows];          // det Z = e^{i pi tsum}
-        load_basis(M, R, st, off, 1, b);
+        float oprev[3][3];           // kFormEigenLocal: the previous shell's eigenbasis
+        LayerTable tn;               // kFormEigenLocal: the next shell's entries, loaded one shell ahead
+        auto basis = [&](int i) {
+            if constexpr (kForm == kFormEigenTable) {
+                load_basis(M, R, st, off, i, b);
+            } else {
+                if (i == 1) load_table(Q, R, st, off, 1, tn);
+                t = tn;
+                // in flight while this shell is computed (the batched kernel's table slices miss
+                // L2 more often than the single grid's)
+                if (i < L) load_table(Q, R, st, off, i + 1, tn);
+                local_basis(t, i == 1, oprev, b);
+            }
+        };
+        basis(1);
 #pragma unroll
         for (int k = 0; k < kRows; k++) {           // Z = D_1 M_1
             tsum[k] = 0.0;
@@ -430,7 +469,7 @@ __device__ inline void propagate_rows_mixed(int L, const double* const* gl, cons
             }
         }
         auto shell = [&](int i) {
-            load_basis(M, R, st, off, i, b);
+            basis(i);
 #pragma unroll
             for (int k = 0; k < kRows; k++) {
                 const LayerDiag d = layer_diag(b.r, gl[k][i], tsum[k]);
@@ -446,7 +485,7 @@ __device__ inline void propagate_rows_mixed(int L, const double* const* gl, cons
 #pragma unroll 1
             for (int i = 2; i < L; i++) shell(i);
         }
-        load_basis(M, R, st, off, L, b);
+        basis(L);
         LayerDiag dL[kRows];
 #pragma unroll
         for (int k = 0; k < kRows; k++) {
@@ -488,67 +527,6 @@ __device__ inline void propagate_rows_mixed(int L, const double* const* gl, cons
                     cmac(acc, h[x][2], u[2][y]);
                     T[x][y] = acc;
                 }
-            block_probabilities(T, p[k]);
-        }
-    } else {
-        Cx<float> Y[kRows][3][3];   // S_{i} ... S_1
-        load_table(Q, R, st, off, 1, t);
-#pragma unroll
-        for (int k = 0; k < kRows; k++) init_layer(t, layer_phases(t, gl[k][1]), Y[k]);
-        auto shell = [&](int i) {
-            load_table(Q, R, st, off, i, t);
-#pragma unroll
-            for (int k = 0; k < kRows; k++) apply_layer<3>(t, layer_phases(t, gl[k][i]), Y[k]);
-        };
-        if (kRows == 1) {
-#pragma unroll 2
-            for (int i = 2; i < L; i++) shell(i);
-        } else {
-#pragma unroll 1
-            for (int i = 2; i < L; i++) shell(i);
-        }
-        load_table(Q, R, st, off, 0, t);
-#pragma unroll
-        for (int k = 0; k < kRows; k++) {
-            atmosphere_columns(t, gl[k][0], wmu, G[k]);
-            Cx<float> N[3][2];                      // G <- Y G
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++) {
-                    Cx<float> acc = cmul(Y[k][a][0], G[k][0][b]);
-                    cmac(acc, Y[k][a][1], G[k][1][b]);
-                    cmac(acc, Y[k][a][2], G[k][2][b]);
-                    N[a][b] = acc;
-                }
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++) G[k][a][b] = N[a][b];
-        }
-        load_table(Q, R, st, off, L, t);
-#pragma unroll
-        for (int k = 0; k < kRows; k++) apply_layer<2>(t, layer_phases(t, gl[k][L]), G[k]);
-#pragma unroll
-        for (int k = 0; k < kRows; k++) {
-            // T[out][in] for out, in in (e, mu): rows e, mu of W Y^T times G
-            Cx<float> He[3], Hm[3], T[2][2];
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                He[c] = Y[k][c][0];
-                Hm[c] = cmul(w1, Y[k][c][1]);
-                cmac(Hm[c], w2, Y[k][c][2]);
-            }
-#pragma unroll
-            for (int b = 0; b < 2; b++) {
-                Cx<float> e = cmul(He[0], G[k][0][b]), m = cmul(Hm[0], G[k][0][b]);
-                cmac(e, He[1], G[k][1][b]);
-                cmac(m, Hm[1], G[k][1][b]);
-                cmac(e, He[2], G[k][2][b]);
-                cmac(m, Hm[2], G[k][2][b]);
-                T[0][b] = e;
-                T[1][b] = m;
-            }
             block_probabilities(T, p[k]);
         }
     }

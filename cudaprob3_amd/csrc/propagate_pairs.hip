@@ -236,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void pairs_mixed_kernel(MixedArgs a) {
     if (n_rows == 2 && crossed_shells(a.maxlayers, ic0 + 1, a.rows) == L0) {
         const double* gl[2] = {a.geom + std::size_t(ic0) * (a.rows + 1), a.geom + std::size_t(ic0 + 1) * (a.rows + 1)};
         float p[2][3][3];
-        propagate_rows_mixed<2, true>(L0, gl, a.Q, a.M, a.R, st, ie, a.wmu, &a.vac, p);
+        propagate_rows_mixed<2, kFormEigenTable>(L0, gl, a.Q, a.M, a.R, st, ie, a.wmu, &a.vac, p);
         emit(ic0, p[0]);
         emit(ic0 + 1, p[1]);
     } else {
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(kBlock) void pairs_mixed_kernel(MixedArgs a) {
             const int ic = ic0 + k;
             const double* gl[1] = {a.geom + std::size_t(ic) * (a.rows + 1)};
             float p[1][3][3];
-            propagate_rows_mixed<1, true>(crossed_shells(a.maxlayers, ic, a.rows), gl, a.Q, a.M, a.R, st, ie, a.wmu,
+            propagate_rows_mixed<1, kFormEigenTable>(crossed_shells(a.maxlayers, ic, a.rows), gl, a.Q, a.M, a.R, st, ie, a.wmu,
                                           &a.vac, p);
             emit(ic, p[0]);
         }

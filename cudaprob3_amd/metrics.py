@@ -10,18 +10,21 @@ min/max and bit operations not counted:
     - L = 0: single-grid kernels: the vacuum block from the constants U_xk conj(U_yk) (64);
       batched kernel: the atmosphere columns G (76) and the block from G (28);
     - L = 1: G (76), the innermost shell on G as two rank-1 terms (112) and the block (28);
-    - L >= 2, eigenbasis form (single-grid kernels, two columns carried): the first shell
+    - L >= 2, eigenbasis form (two columns carried; single-grid kernels with the basis changes
+      from the table, form "table", batched kernel forming them, form "local"): the first shell
       D_1 M_1 (8), per further shell of the way in one basis change and phase scaling of two
       columns (84) and two FP64 phase sums, the innermost basis change (60), the completion of
       Z' (normalised columns 34, cross product 42, times the determinant 18, and the
       determinant's phase factor 22 + 4 FP64), G (76), h = D_L Z' a_x (66), u = Z' G (132) and
-      the block (88);
-    - L >= 2, rank-1 form (batched kernel): the first shell (54), per further shell two rank-1
+      the block (88); form "local" adds per cosine row, or per pair of rows with the same L
+      propagated together, 9 flops for M_1 = O_1^T and 54 for each further basis change
+      (O_s = [v_1 x v_2, v_1, v_2] and nine dot products);
+    - L >= 2, rank-1 form: the first shell (54), per further shell two rank-1
       terms on the 3x3 running product (168), G (76), Y G (132), the innermost shell on two
       columns (112) and the block through W Y^T (130).
-  FP32 per pair: L = 0: 132 (single-grid) or 172 (batched), L = 1: 330, L >= 2: 128 L + 446
-  (eigenbasis) or 214 L + 236 (rank-1); FP64: 8 (L + 1), plus 2 L + 2 for L >= 2 in the
-  eigenbasis form.  This is the roofline's numerator; it agrees within a few per cent with the
+  FP32 per pair: L = 0: 132 (form "table") or 172, L = 1: 330, L >= 2: 128 L + 446 (eigenbasis,
+  plus the basis changes of form "local") or 214 L + 236 (rank-1); FP64: 8 (L + 1), plus 2 L + 2
+  for L >= 2 in the eigenbasis form.  This is the roofline's numerator; it agrees within a few per cent with the
   FP32/FP64 flops that rocprofv3's SQ_INSTS_VALU_{ADD,MUL,FMA}_F{32,64} counters report.
 * The reference algorithm as written (physics.hpp:268-702), kept as a measure of the work the
   reference would do for the same results (``reference_work``): one layer step
@@ -63,6 +66,8 @@ EIGEN_SHELL = 84              # Z <- D_i M_i Z
 EIGEN_SHELL_F64 = 2           # the determinant's phase sum
 EIGEN_INNERMOST = 60          # Z' = M_L Z
 EIGEN_COMPLETE = 94           # normalised columns, c_0 x c_1 times the determinant
+LOCAL_BASIS_FIRST = 9         # form "local": O_1 and M_1 = O_1^T
+LOCAL_BASIS = 54              # form "local": O_s and M_s = O_s^T O_{s-1}
 EIGEN_TAIL = 66 + 132 + 88    # h = D_L Z' a_x, u = Z' G, the 2x2 block
 # rank-1 form (L >= 2)
 RANK1_FIRST_SHELL = 54        # init_layer
@@ -72,15 +77,21 @@ RANK1_BLOCK = 130             # the 2x2 block through W Y^T
 LIKELIHOOD_F64_PER_PAIR = 27  # nine (P - D)^2 accumulated in FP64 (likelihood variant)
 
 
-def mixed_flops_per_pair(L, eigenbasis: bool = True):
-    """(FP32, FP64) flops of the mixed pair kernels for a pair crossing L shells: the single-grid
-    kernels (eigenbasis form) or the batched kernel (rank-1 form)."""
+FORMS = ("table", "local", "rank1")
+
+
+def mixed_flops_per_pair(L, form: str = "table"):
+    """(FP32, FP64) flops of the mixed pair kernels for a pair crossing L shells, without form
+    "local"'s basis changes (counted per row group, local_basis_flops): form "table" for the
+    single-grid kernels, "local" for the batched kernel, "rank1" for the rank-1 algebra."""
+    if form not in FORMS:
+        raise ValueError(f"form must be one of {FORMS}")
     L = np.asarray(L, dtype=np.int64)
     shallow = (2 * MIXED_PHASE_F32 + MIXED_ATMOSPHERE + MIXED_BLOCK_SHALLOW + MIXED_PROBS +
                np.where(L == 1, 2 * MIXED_PHASE_F32 + MIXED_INNERMOST, 0))
-    if eigenbasis:
+    if form == "table":
         shallow = np.where(L == 0, 2 * MIXED_PHASE_F32 + MIXED_VACUUM_BLOCK + MIXED_PROBS, shallow)
-    if eigenbasis:
+    if form != "rank1":
         deep = (2 * MIXED_PHASE_F32 + 2 * MIXED_DIAG_F32 * L + EIGEN_FIRST_SHELL + EIGEN_SHELL * (L - 2) +
                 EIGEN_INNERMOST + EIGEN_COMPLETE + MIXED_DIAG_F32 + MIXED_ATMOSPHERE + EIGEN_TAIL + MIXED_PROBS)
     else:
@@ -88,18 +99,33 @@ def mixed_flops_per_pair(L, eigenbasis: bool = True):
                 RANK1_Y_TIMES_G + MIXED_INNERMOST + RANK1_BLOCK + MIXED_PROBS)
     f32 = np.where(L >= 2, deep, shallow)
     f64 = 2 * MIXED_PHASE_F64 * (L + 1)
-    if eigenbasis:
+    if form != "rank1":
         f64 = f64 + np.where(L >= 2, EIGEN_SHELL_F64 * (L - 1) + MIXED_PHASE_F64, 0)
     return f32, f64
 
 
+def local_basis_flops(maxlayers, n_energies: int) -> float:
+    """Form "local"'s basis changes over a grid: the batched kernel takes cosine rows in pairs
+    (2j, 2j + 1) and forms them once for a pair whose rows cross the same L >= 2 shells, once per
+    row otherwise (an unpaired last row included)."""
+    L = np.asarray(maxlayers, dtype=np.int64)
+    per_group = lambda x: np.where(x >= 2, LOCAL_BASIS_FIRST + LOCAL_BASIS * (x - 1), 0)
+    n = len(L) // 2 * 2
+    a, b = L[0:n:2], L[1:n:2]
+    groups = np.where(a == b, per_group(a), per_group(a) + per_group(b)).sum()
+    if len(L) % 2:
+        groups += per_group(L[-1])
+    return float(groups) * n_energies
+
+
 def mixed_grid_flops(maxlayers, n_energies: int, likelihood: bool = False,
-                     eigenbasis: bool = True) -> tuple[float, float]:
+                     form: str = "table") -> tuple[float, float]:
     """(FP32, FP64) flops of one mixed pair-kernel launch over a grid."""
-    f32, f64 = mixed_flops_per_pair(maxlayers, eigenbasis)
+    f32, f64 = mixed_flops_per_pair(maxlayers, form)
     n_pairs = len(np.asarray(maxlayers)) * n_energies
     extra = LIKELIHOOD_F64_PER_PAIR * n_pairs if likelihood else 0
-    return float(f32.sum()) * n_energies, float(f64.sum()) * n_energies + extra
+    local = local_basis_flops(maxlayers, n_energies) if form == "local" else 0.0
+    return float(f32.sum()) * n_energies + local, float(f64.sum()) * n_energies + extra
 
 
 def valu_roofline(f32_flops: float, f64_flops: float, kernel_ms: float) -> dict:

@@ -68,22 +68,23 @@ def test_mixed_flops_agree_with_the_committed_counters():
     f32, f64 = M.mixed_grid_flops(ml, len(g.energies), likelihood=True)
     counted = _profile("traffic_mcmc8192_mixed.json")["counter_flops_per_launch"]
     assert abs((f32 + f64) / counted - 1) < 0.03
-    # the batched kernel's rank-1 form does not describe this kernel
-    f32r, f64r = M.mixed_grid_flops(ml, len(g.energies), likelihood=True, eigenbasis=False)
+    # the rank-1 algebra does not describe this kernel
+    f32r, f64r = M.mixed_grid_flops(ml, len(g.energies), likelihood=True, form="rank1")
     assert abs((f32r + f64r) / counted - 1) > 0.1
 
 
 def test_batched_flops_agree_with_the_committed_counters():
-    """The batched kernel (rank-1 form) over the 1024 scan sets: the counters also count the lanes
+    """The batched kernel (eigenbasis form, basis changes formed per row group) over the 1024 scan
+    sets: the counters also count the lanes
     of each set's last 64-energy tile beyond its energies (masked lanes still issue), so the
     algorithmic count with energies padded to 64 must match them within 3 %; the roofline uses the
     unpadded count (useful work only), about 5 % lower."""
     useful = padded = 0.0
     for s in W.scan_sets():
         ml = M.maxlayers_of(s.cosines, s.radii, s.rhos)
-        a, b = M.mixed_grid_flops(ml, len(s.energies), eigenbasis=False)
+        a, b = M.mixed_grid_flops(ml, len(s.energies), form="local")
         useful += a + b
-        a, b = M.mixed_grid_flops(ml, -(-len(s.energies) // 64) * 64, eigenbasis=False)
+        a, b = M.mixed_grid_flops(ml, -(-len(s.energies) // 64) * 64, form="local")
         padded += a + b
     counted = _profile("traffic_scan1024_mixed.json")["counter_flops_per_launch"]
     assert abs(padded / counted - 1) < 0.03

@@ -1,8 +1,9 @@
 """CPU emulation of the mixed-precision algebra of the GPU kernels (pair_mixed.hpp) against the
 reference's golden vectors: pins the precision design itself, independently of a GPU.  Two forms
-of the running product are emulated: the rank-1 form of the batched kernel (emulate) and the
-eigenbasis form of the single-grid kernels for rows crossing two or more shells
-(emulate_eigenbasis).
+of the running product are emulated: rank-1 updates (emulate; the kernels' form for rows crossing
+at most one shell, and the earlier form of every row) and the eigenbasis form of rows crossing two
+or more shells, with the table's basis changes (single-grid kernels) or with basis changes formed
+from the float eigenvectors (batched kernel) (emulate_eigenbasis).
 
 What is emulated, stage by stage (numpy, vectorised over the energies of a cosine row):
   * the real basis (host_util.hpp real_basis): the unitary W = diag(1, W') that makes the vacuum
@@ -256,13 +257,31 @@ def cross3(a, b):
                      a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]], axis=1)
 
 
-def emulate_eigenbasis(g, shuffle_m=False, det_phase=True):
+def local_basis_tables(tables):
+    """The batched kernel's basis changes, formed in float32 from the stored float32 v_1, v_2:
+    O_s = [v_1 x v_2, v_1, v_2], M_s = O_s^T O_{s-1}, M_1 = O_1^T."""
+    out, prev = [], None
+    for slot, (rate, v, _) in enumerate(tables):
+        if slot == 0:
+            out.append((rate, v, None))
+            continue
+        v1, v2 = v[0].astype(F32), v[1].astype(F32)
+        O = np.stack([cross3(v1, v2).astype(F32), v1, v2], axis=2)          # [e][component][column]
+        M = np.transpose(O, (0, 2, 1)).copy() if prev is None else np.einsum("eak,eac->ekc", O, prev).astype(F32)
+        out.append((rate, v, M))
+        prev = O
+    return out
+
+
+def emulate_eigenbasis(g, shuffle_m=False, det_phase=True, batched=False):
     """The single-grid mixed kernel: for rows with L >= 2 the first two columns of the running
     product Z = D_i M_i ... D_1 M_1 (M real 3x3 basis changes, D_i = diag(1, e^{i dphi_1},
     e^{i dphi_2})) and the FP64 sum of the phases (half turns), then the first two columns of
     Z' = M_L Z, normalised, completed by det(Z') conj(c_0 x c_1) with det = e^{i pi sum}, and
     T[x][y] = sum_c (Z' a_x)_c D_L,c (Z' G)_c,y with a_e = e_0, a_mu = row mu of W; for L = 0 the
-    vacuum form T = I + sum_{k=1,2} w_k U_xk conj(U_yk); L = 1 as in emulate()."""
+    vacuum form T = I + sum_{k=1,2} w_k U_xk conj(U_yk); L = 1 as in emulate().  batched: the
+    batched kernel's variant, basis changes formed from the float eigenvectors
+    (local_basis_tables) and L = 0 as in emulate()."""
     p = Propagator(len(g["cosines"]), len(g["energies"]))
     p.setEnergyList(g["energies"])
     p.setCosineList(g["cosines"])
@@ -279,6 +298,8 @@ def emulate_eigenbasis(g, shuffle_m=False, det_phase=True):
     Wr = W[:2, :].astype(np.complex64)
     rhos = [0.0] + [float(r) * 0.5 for r in p.rhos]
     tables = eigenbasis_tables(Hr, E, rhos, antinu)
+    if batched:
+        tables = local_basis_tables(tables)
     base = emulate(g)
     # vacuum constants (W v_k)_x conj((W v_k)_y) with the table's labelling of the eigenvectors (for
     # the kernel's labelling, by mass index, these are U_xk conj(U_yk))
@@ -289,7 +310,7 @@ def emulate_eigenbasis(g, shuffle_m=False, det_phase=True):
     for ic, cz in enumerate(p.cosineList):
         L = int(p.maxlayers[ic])
         dist = oracle.layer_distances(p.radii, L, cz, float(p.ProductionHeightinCentimeter)) / 1e5
-        if L == 0:       # one vacuum layer: T = I + sum_k w_k U_xk conj(U_yk)
+        if L == 0 and not batched:       # one vacuum layer: T = I + sum_k w_k U_xk conj(U_yk)
             w = [phase_factor_m1(tables[0][0][k] * dist[0]) for k in (0, 1)]
             T = (np.eye(2, dtype=np.complex64)[None] + w[0][:, None, None] * vac[0][None] +
                  w[1][:, None, None] * vac[1][None]).astype(np.complex64)
@@ -306,7 +327,6 @@ def emulate_eigenbasis(g, shuffle_m=False, det_phase=True):
         if L < 2:
             out[ic] = base[ic]
             continue
-
         tsum = np.zeros(n_e)
 
         def d(i, add=True):
@@ -355,14 +375,16 @@ def emulate_eigenbasis(g, shuffle_m=False, det_phase=True):
     return out
 
 
+@pytest.mark.parametrize("batched", [False, True])
 @pytest.mark.parametrize("name", golden_names())
-def test_eigenbasis_algebra_meets_tolerance(name):
-    """The eigenbasis form of the running product (rows with L >= 2) against the reference's
-    golden values, within the mixed bar."""
+def test_eigenbasis_algebra_meets_tolerance(name, batched):
+    """The eigenbasis form of the running product (rows with L >= 2), with the table's basis
+    changes (single-grid kernels) or with basis changes formed from the float eigenvectors
+    (batched kernel), against the reference's golden values, within the mixed bar."""
     g = load_golden(name)
-    P = emulate_eigenbasis(g)
+    P = emulate_eigenbasis(g, batched=batched)
     err = np.abs(P - g["P"]).max()
-    print(f"{name}: emulated eigenbasis algebra max|dP| {err:.2e}")
+    print(f"{name} batched={batched}: emulated eigenbasis algebra max|dP| {err:.2e}")
     assert err <= MIXED_TOL
 
 
@@ -395,6 +417,7 @@ def test_eigenbasis_long_paths_on_a_100_shell_earth():
         g = dict(cosines=cos, energies=E, radii=radii[::-1].copy(), rhos=rhos[::-1].copy(), h_km=22.0,
                  params=np.array(params), antinu=antinu)
         g["P"] = oracle.propagate(antinu, cos, E, g["radii"], g["rhos"], 22.0, params)
-        err = np.abs(emulate_eigenbasis(g) - g["P"]).max()
-        print(f"fine100 antinu={antinu} dm32={dm32}: emulated max|dP| {err:.2e}")
-        assert err <= 0.5 * MIXED_TOL
+        for batched in (False, True):
+            err = np.abs(emulate_eigenbasis(g, batched=batched) - g["P"]).max()
+            print(f"fine100 antinu={antinu} dm32={dm32} batched={batched}: emulated max|dP| {err:.2e}")
+            assert err <= 0.5 * MIXED_TOL
