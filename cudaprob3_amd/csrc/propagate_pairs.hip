@@ -15,19 +15,19 @@
 //  * A real basis.  The flavour Hamiltonian of every layer is W H_r W^dagger with H_r real
 //    symmetric and one unitary W per hypothesis (W = diag(1, W'), it commutes with the
 //    electron-only matter term; host_util.hpp real_basis).  So the layer amplitudes are
-//    W S W^dagger with S = sum_k e^{i phi_k} v_k v_k^T for real eigenvectors v_k: complex
-//    symmetric.  The mantle product of the way out is then the transpose of the product of the
-//    way in, and one running product serves both (pair_mixed.hpp).
-//  * The layer's global phase e^{i phi_0} is dropped (it cancels in |amplitude|^2), and
-//    sum_k v_k v_k^T = I gives S' = I + (e^{i dphi_1} - 1) v_1 v_1^T + (e^{i dphi_2} - 1) v_2 v_2^T
-//    with dphi_k = phi_k - phi_0: two real rank-1 updates per layer.  dphi_k (up to ~10^3 rad
-//    on core-crossing paths at 0.05 GeV, where FP32 would lose 1e-4 rad) is formed and reduced
-//    to [-pi/4, pi/4] plus a quadrant in FP64; sin/cos of the remainder (polynomials,
-//    pair_mixed.hpp) and the matrix arithmetic are FP32.
+//    W S W^dagger with S = O D O^T for the layer's real eigenbasis O: complex symmetric, and the
+//    mantle product of the way out is the transpose of the product of the way in, so one running
+//    product serves both (pair_mixed.hpp).
+//  * The layer's global phase is dropped (it cancels in |amplitude|^2): D = diag(1, e^{i dphi_1},
+//    e^{i dphi_2}).  The running product is carried in the shells' eigenbases, two of its three
+//    columns, and completed from its determinant at the end (pair_mixed.hpp).  dphi_k (up to
+//    ~10^3 rad on core-crossing paths at 0.05 GeV, where FP32 would lose 1e-4 rad) is formed and
+//    reduced to half a turn plus a sign in FP64; sin/cos of the remainder (polynomials) and the
+//    matrix arithmetic are FP32.
 //  * The total amplitude is unitary, so its upper-left 2x2 block fixes all nine
 //    probabilities (rows and columns of |T|^2 sum to one).
-//  * v_1, v_2 are read as two coalesced float4 loads per lane and layer.
-//
+//  * A path that crosses no shell is one vacuum layer: T = U D U^dagger from host constants.
+
 // Mapping: one wavefront = two cosine rows x 64 consecutive energies (lane = energy); the four
 // wavefronts of a workgroup take eight consecutive rows of the same energy tile, and
 // consecutive logical workgroups stay on one XCD so that an energy tile's table slice is

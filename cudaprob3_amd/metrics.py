@@ -128,6 +128,17 @@ def mixed_grid_flops(maxlayers, n_energies: int, likelihood: bool = False,
     return float(f32.sum()) * n_energies + local, float(f64.sum()) * n_energies + extra
 
 
+def f64_hoisted_flops_per_pair(L):
+    """FP64 flops of the hoisted FP64 pair kernel (propagate_pairs.hip pairs_f64_kernel) for a
+    pair crossing L shells, as written (add/sub/mul/div = 1; sincos excluded): per layer the
+    three phase arguments r L / E (6) and A = sum_k e^{i phi_k} Q_k from the packed Hermitian
+    projectors (138); the way-in product fin = A fin for layers 1..L and the mantle product for
+    layers 2..L-1 and the final mantle fin for L >= 2 (198 each, a 3x3 complex product); the nine
+    |amplitude|^2 (27).  F(0) = 171, F(1) = 513, F(L >= 2) = 540 L - 27."""
+    L = np.asarray(L, dtype=np.int64)
+    return np.where(L == 0, 171, np.where(L == 1, 513, 540 * L - 27))
+
+
 def valu_roofline(f32_flops: float, f64_flops: float, kernel_ms: float) -> dict:
     """Roofline entries of a vector-ALU kernel: achieved = all flops / time against the FP32
     vector peak (FP64 flops are counted at the FP32 rate, which understates the fraction by
