@@ -40,23 +40,6 @@
 namespace cp3 {
 namespace {
 
-template <typename T>
-__device__ inline void mul3(const Cx<T> A[3][3], const Cx<T> B[3][3], Cx<T> C[3][3]) {
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            T re = A[i][0].re * B[0][j].re - A[i][0].im * B[0][j].im;
-            T im = A[i][0].im * B[0][j].re + A[i][0].re * B[0][j].im;
-#pragma unroll
-            for (int k = 1; k < 3; k++) {
-                re += A[i][k].re * B[k][j].re - A[i][k].im * B[k][j].im;
-                im += A[i][k].im * B[k][j].re + A[i][k].re * B[k][j].im;
-            }
-            C[i][j] = Cx<T>{re, im};
-        }
-}
-
 // A = sum_k (c_k + i s_k) Q_k for Hermitian Q_k packed as (Q00, Q11, Q22, Q01, Q02, Q12).
 template <typename T>
 __device__ inline void assemble(const T q[27], const T c[3], const T s[3], Cx<T> A[3][3]) {
@@ -98,24 +81,49 @@ struct PairArgs {
     int n_c, n_e, rows, ne_pad;
 };
 
-// FP64 pair kernel: R = -2.534 lambda_k, phase = R * L / E.
+// sum_k a[k] b[k] for complex 3-vectors (a row times a column), in the reference's order
+// (math.hpp:50-64: the k = 0 term, then k = 1, 2 added).
+template <typename T>
+__device__ inline Cx<T> cdot3(const Cx<T> a[3], const Cx<T> b[3]) {
+    T re = a[0].re * b[0].re - a[0].im * b[0].im;
+    T im = a[0].im * b[0].re + a[0].re * b[0].im;
+#pragma unroll
+    for (int k = 1; k < 3; k++) {
+        re += a[k].re * b[k].re - a[k].im * b[k].im;
+        im += a[k].im * b[k].re + a[k].re * b[k].im;
+    }
+    return Cx<T>{re, im};
+}
+
+// FP64 pair kernel: R = -2.534 lambda_k, phase = R * L / E.  The way-in product `fin` stays in
+// registers and is updated in place a column at a time; the mantle product of every lane waits
+// in LDS (park[e][lane], e = (row * 3 + column) * 2 + {re, im}; each lane touches only its own
+// column, so no barrier is needed).  With both products and a temporary in registers the kernel
+// held 184 registers per lane (two wavefronts per SIMD).
 __global__ __launch_bounds__(kBlock) void pairs_f64_kernel(PairArgs a) {
     using T = double;
+    __shared__ T park[18][kBlock];
     const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
     const int tiles = (a.n_e + kWave - 1) / kWave;
     const int row_groups = (a.n_c + 3) / 4;
     const int tile = lb / row_groups;
-    const int ic = (lb % row_groups) * 4 + threadIdx.x / kWave;
+    const int ic = __builtin_amdgcn_readfirstlane((lb % row_groups) * 4 + int(threadIdx.x) / kWave);
     if (tile >= tiles || ic >= a.n_c) return;
     const int ie = tile * kWave + threadIdx.x % kWave;   // < ne_pad: table reads stay in bounds
 
-    const int L = a.maxlayers[ic];
+    const int L = __builtin_amdgcn_readfirstlane(a.maxlayers[ic]);
     const double* gl = a.geom + std::size_t(ic) * (a.rows + 1);
     const T* Q = static_cast<const T*>(a.Q);
     const std::size_t st = a.ne_pad;
     const double E = a.energies[ie < a.n_e ? ie : a.n_e - 1];
+    const int lane = int(threadIdx.x);
+    auto mantle_get = [&](int x, int y) { return Cx<T>{park[(x * 3 + y) * 2][lane], park[(x * 3 + y) * 2 + 1][lane]}; };
+    auto mantle_put = [&](int x, int y, Cx<T> v) {
+        park[(x * 3 + y) * 2][lane] = v.re;
+        park[(x * 3 + y) * 2 + 1][lane] = v.im;
+    };
 
-    Cx<T> fin[3][3], mantle[3][3], A[3][3], tmp[3][3];
+    Cx<T> fin[3][3], A[3][3];
     for (int i = 0; i <= L; i++) {
         const double dist = gl[i];
         const T* q = Q + std::size_t(i) * 27 * st + ie;
@@ -135,34 +143,54 @@ __global__ __launch_bounds__(kBlock) void pairs_f64_kernel(PairArgs a) {
             for (int x = 0; x < 3; x++)
 #pragma unroll
                 for (int y = 0; y < 3; y++) fin[x][y] = A[x][y];
-        } else {
-            mul3<T>(A, fin, tmp);
+            continue;
+        }
+        // fin <- A fin
 #pragma unroll
-            for (int x = 0; x < 3; x++)
+        for (int y = 0; y < 3; y++) {
+            const Cx<T> colv[3] = {fin[0][y], fin[1][y], fin[2][y]};
+            Cx<T> out[3];
 #pragma unroll
-                for (int y = 0; y < 3; y++) fin[x][y] = tmp[x][y];
-            if (i < L) {
-                if (i == 1) {
+            for (int x = 0; x < 3; x++) out[x] = cdot3<T>(A[x], colv);
 #pragma unroll
-                    for (int x = 0; x < 3; x++)
+            for (int x = 0; x < 3; x++) fin[x][y] = out[x];
+        }
+        if (i >= L) continue;
+        // mantle <- mantle A (the first mantle layer is A itself)
 #pragma unroll
-                        for (int y = 0; y < 3; y++) mantle[x][y] = A[x][y];
-                } else {
-                    mul3<T>(mantle, A, tmp);
+        for (int x = 0; x < 3; x++) {
+            if (i == 1) {
 #pragma unroll
-                    for (int x = 0; x < 3; x++)
-#pragma unroll
-                        for (int y = 0; y < 3; y++) mantle[x][y] = tmp[x][y];
-                }
+                for (int y = 0; y < 3; y++) mantle_put(x, y, A[x][y]);
+                continue;
             }
+            const Cx<T> rowv[3] = {mantle_get(x, 0), mantle_get(x, 1), mantle_get(x, 2)};
+            Cx<T> out[3];
+#pragma unroll
+            for (int y = 0; y < 3; y++) {
+                const Cx<T> colv[3] = {A[0][y], A[1][y], A[2][y]};
+                out[y] = cdot3<T>(rowv, colv);
+            }
+#pragma unroll
+            for (int y = 0; y < 3; y++) mantle_put(x, y, out[y]);
         }
     }
     if (L >= 2) {
-        mul3<T>(mantle, fin, tmp);
+        // fin <- mantle fin
+        Cx<T> mt[3][3];
 #pragma unroll
         for (int x = 0; x < 3; x++)
 #pragma unroll
-            for (int y = 0; y < 3; y++) fin[x][y] = tmp[x][y];
+            for (int y = 0; y < 3; y++) mt[x][y] = mantle_get(x, y);
+#pragma unroll
+        for (int y = 0; y < 3; y++) {
+            const Cx<T> colv[3] = {fin[0][y], fin[1][y], fin[2][y]};
+            Cx<T> out[3];
+#pragma unroll
+            for (int x = 0; x < 3; x++) out[x] = cdot3<T>(mt[x], colv);
+#pragma unroll
+            for (int x = 0; x < 3; x++) fin[x][y] = out[x];
+        }
     }
     if (ie >= a.n_e) return;
     const std::size_t plane = std::size_t(a.n_c) * a.n_e;
