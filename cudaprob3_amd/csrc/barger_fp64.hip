@@ -640,8 +640,8 @@ __device__ inline int find_set(const long long* __restrict__ off, int n, long lo
 }
 
 // Batched set-up of many parameter sets in one launch: work cells [0, G) are layer lengths
-// (set, cosine, layer), [G, G + S) spectrum cells (set, slot, energy) of the rank-1 tables (the
-// batched kernel needs no basis changes).  Also resets the pair kernel's work queue.
+// (set, cosine, layer), [G, G + S) spectrum cells (set, slot, energy) without basis changes (the
+// batched pair kernel forms them from the eigenvectors).  Also resets the pair kernel's work queue.
 __global__ __launch_bounds__(kBlock) void batch_setup(BatchDesc b) {
     const long long G = b.geom_prefix[b.n_sets], S = b.spec_prefix[b.n_sets];
     if (blockIdx.x == 0 && threadIdx.x < kXcds) b.next_item[threadIdx.x * kQueueStride] = 0;   // pair kernel follows

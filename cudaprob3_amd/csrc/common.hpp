@@ -164,7 +164,7 @@ struct BatchDesc {
     const double* radii;
     const double* rhos;
     double* geom;
-    float4* Q;                      // kMixedQGroups float4 per spectrum cell (rank-1 tables: no basis changes)
+    float4* Q;                      // kMixedQGroups float4 per spectrum cell (no basis changes: the pair kernel forms them)
     double2* R;
     unsigned long long* next_item;  // kXcds work-queue counters of the pair kernel, kQueueStride apart
                                     // (zeroed by the set-up kernel)

@@ -19,12 +19,8 @@ min/max and bit operations not counted:
       the block (88); form "local" adds per cosine row, or per pair of rows with the same L
       propagated together, 9 flops for M_1 = O_1^T and 54 for each further basis change
       (O_s = [v_1 x v_2, v_1, v_2] and nine dot products);
-    - L >= 2, rank-1 form: the first shell (54), per further shell two rank-1
-      terms on the 3x3 running product (168), G (76), Y G (132), the innermost shell on two
-      columns (112) and the block through W Y^T (130).
-  FP32 per pair: L = 0: 132 (form "table") or 172, L = 1: 330, L >= 2: 128 L + 446 (eigenbasis,
-  plus the basis changes of form "local") or 214 L + 236 (rank-1); FP64: 8 (L + 1), plus 2 L + 2
-  for L >= 2 in the eigenbasis form.  This is the roofline's numerator; it agrees within a few per cent with the
+  FP32 per pair: L = 0: 132 (form "table") or 172 (form "local"), L = 1: 330, L >= 2: 128 L + 446
+  (plus the basis changes of form "local"); FP64: 8 (L + 1), plus 2 L + 2 for L >= 2.  This is the roofline's numerator; it agrees within a few per cent with the
   FP32/FP64 flops that rocprofv3's SQ_INSTS_VALU_{ADD,MUL,FMA}_F{32,64} counters report.
 * The reference algorithm as written (physics.hpp:268-702), kept as a measure of the work the
   reference would do for the same results (``reference_work``): one layer step
@@ -69,21 +65,16 @@ EIGEN_COMPLETE = 94           # normalised columns, c_0 x c_1 times the determin
 LOCAL_BASIS_FIRST = 9         # form "local": O_1 and M_1 = O_1^T
 LOCAL_BASIS = 54              # form "local": O_s and M_s = O_s^T O_{s-1}
 EIGEN_TAIL = 66 + 132 + 88    # h = D_L Z' a_x, u = Z' G, the 2x2 block
-# rank-1 form (L >= 2)
-RANK1_FIRST_SHELL = 54        # init_layer
-RANK1_SHELL = 168             # 3x3 running product, two rank-1 terms (apply_layer<3>)
-RANK1_Y_TIMES_G = 132         # running product times G
-RANK1_BLOCK = 130             # the 2x2 block through W Y^T
 LIKELIHOOD_F64_PER_PAIR = 27  # nine (P - D)^2 accumulated in FP64 (likelihood variant)
 
 
-FORMS = ("table", "local", "rank1")
+FORMS = ("table", "local")
 
 
 def mixed_flops_per_pair(L, form: str = "table"):
     """(FP32, FP64) flops of the mixed pair kernels for a pair crossing L shells, without form
     "local"'s basis changes (counted per row group, local_basis_flops): form "table" for the
-    single-grid kernels, "local" for the batched kernel, "rank1" for the rank-1 algebra."""
+    single-grid kernels, "local" for the batched kernel."""
     if form not in FORMS:
         raise ValueError(f"form must be one of {FORMS}")
     L = np.asarray(L, dtype=np.int64)
@@ -91,16 +82,10 @@ def mixed_flops_per_pair(L, form: str = "table"):
                np.where(L == 1, 2 * MIXED_PHASE_F32 + MIXED_INNERMOST, 0))
     if form == "table":
         shallow = np.where(L == 0, 2 * MIXED_PHASE_F32 + MIXED_VACUUM_BLOCK + MIXED_PROBS, shallow)
-    if form != "rank1":
-        deep = (2 * MIXED_PHASE_F32 + 2 * MIXED_DIAG_F32 * L + EIGEN_FIRST_SHELL + EIGEN_SHELL * (L - 2) +
-                EIGEN_INNERMOST + EIGEN_COMPLETE + MIXED_DIAG_F32 + MIXED_ATMOSPHERE + EIGEN_TAIL + MIXED_PROBS)
-    else:
-        deep = (2 * MIXED_PHASE_F32 * (L + 1) + RANK1_FIRST_SHELL + RANK1_SHELL * (L - 2) + MIXED_ATMOSPHERE +
-                RANK1_Y_TIMES_G + MIXED_INNERMOST + RANK1_BLOCK + MIXED_PROBS)
+    deep = (2 * MIXED_PHASE_F32 + 2 * MIXED_DIAG_F32 * L + EIGEN_FIRST_SHELL + EIGEN_SHELL * (L - 2) +
+            EIGEN_INNERMOST + EIGEN_COMPLETE + MIXED_DIAG_F32 + MIXED_ATMOSPHERE + EIGEN_TAIL + MIXED_PROBS)
     f32 = np.where(L >= 2, deep, shallow)
-    f64 = 2 * MIXED_PHASE_F64 * (L + 1)
-    if form != "rank1":
-        f64 = f64 + np.where(L >= 2, EIGEN_SHELL_F64 * (L - 1) + MIXED_PHASE_F64, 0)
+    f64 = 2 * MIXED_PHASE_F64 * (L + 1) + np.where(L >= 2, EIGEN_SHELL_F64 * (L - 1) + MIXED_PHASE_F64, 0)
     return f32, f64
 
 

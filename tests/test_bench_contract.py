@@ -68,9 +68,10 @@ def test_mixed_flops_agree_with_the_committed_counters():
     f32, f64 = M.mixed_grid_flops(ml, len(g.energies), likelihood=True)
     counted = _profile("traffic_mcmc8192_mixed.json")["counter_flops_per_launch"]
     assert abs((f32 + f64) / counted - 1) < 0.03
-    # the rank-1 algebra does not describe this kernel
-    f32r, f64r = M.mixed_grid_flops(ml, len(g.energies), likelihood=True, form="rank1")
-    assert abs((f32r + f64r) / counted - 1) > 0.1
+    # the batched kernel's form (no vacuum rows, basis changes formed per row group) does not
+    # describe this kernel
+    f32b, f64b = M.mixed_grid_flops(ml, len(g.energies), likelihood=True, form="local")
+    assert abs((f32b + f64b) / counted - 1) > 0.03
 
 
 def test_batched_flops_agree_with_the_committed_counters():

@@ -1,9 +1,10 @@
 """Batched parameter scans (one persistent launch over many sets with uneven grids and Earth
 models) against the single-grid mixed propagator and the oracle (mixed tolerance).  The two
 kernels share the per-pair code of rows that cross exactly one shell (bit for bit there); for
-deeper rows the single-grid kernel carries the running product in each shell's eigenbasis and the
-batched one as rank-1 updates, and for rows that cross no shell the single-grid kernel uses the
-vacuum form, so there they agree within the mixed bar."""
+deeper rows both carry the running product in the shells' eigenbases, with basis changes from
+the table (single grid, FP64-formed) or formed from the float eigenvectors (batched), and for
+rows that cross no shell the single-grid kernel uses the vacuum form, so there they agree within
+the mixed bar."""
 import numpy as np
 import pytest
 
