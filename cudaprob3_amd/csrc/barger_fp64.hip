@@ -422,50 +422,57 @@ __global__ __launch_bounds__(kBlock) void layer_geometry(GridDesc g, double* __r
     geom[t] = geometry_cell(g, t / slots, t % slots);
 }
 
-// Flavour-basis projectors of one (energy, density) layer Hamiltonian:
-// Q_k = U P_k U^dagger (P_k the eq. 11 products), Hermitian, packed as 9 reals per k:
-// (Q00, Q11, Q22, Re Q01, Im Q01, Re Q02, Im Q02, Re Q12, Im Q12).  Also returns the
-// eigenvalue differences mm and mv of matter_masses.
-// Only Q_k for k >= kFirst are formed (the mixed tables drop Q_0).
-template <int kFirst = 0>
-__device__ inline void flavour_projectors(const OscParams& p, double E, double rho, double Q[3][9],
-                                          double mm[3][3], double mv[3][3]) {
+// One flavour-basis projector of one (energy, density) layer Hamiltonian:
+// Q_k = U P_k U^dagger (P_k the eq. 11 product of physics.hpp:395-428 with the (k; a, b)
+// pairing (0; 1, 2), (1; 2, 0), (2; 0, 1)), Hermitian, packed as 9 reals:
+// (Q00, Q11, Q22, Re Q01, Im Q01, Re Q02, Im Q02, Re Q12, Im Q12).  Each entry of P_k is summed
+// in the reference's order (the same operations as projector_elem's entry k).  Also returns the
+// phase coefficient rate = -2.534 lambda_k.
+template <int k>
+__device__ inline void flavour_projector(const OscParams& p, double E, double rho, double Q[9], double& rate) {
+    double mm[3][3], mv[3][3];
     matter_masses(p, E, rho, mm, mv);
     TwoEH h;
     two_eh(p, E, rho, mv, h);
-    Cd P[3][3][3];  // [k][i][j]
+    constexpr int a = k == 0 ? 1 : (k == 1 ? 2 : 0), b = k == 0 ? 2 : (k == 1 ? 0 : 1);
+    const double d = mm[k][a] * mm[k][b];
+    Cd P[3][3];
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            Cd v[3];
-            projector_elem(h, mm, i, j, v);
-            P[0][i][j] = v[0]; P[1][i][j] = v[1]; P[2][i][j] = v[2];
-        }
+            Cd acc{0.0, 0.0};
 #pragma unroll
-    for (int k = kFirst; k < 3; k++) {
-        const int nm[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
-#pragma unroll
-        for (int w = 0; w < 6; w++) {
-            const int n = nm[w][0], m = nm[w][1];
-            Cd a{0.0, 0.0};
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    double f[4];
-                    axfac(p, n, m, i, j, f);
-                    a.re += f[0] * P[k][i][j].re + f[1] * P[k][i][j].im;
-                    a.im += f[2] * P[k][i][j].im + f[3] * P[k][i][j].re;
-                }
-            if (w < 3) {
-                Q[k][w] = a.re;
-            } else {
-                Q[k][3 + 2 * (w - 3)] = a.re;
-                Q[k][4 + 2 * (w - 3)] = a.im;
+            for (int l = 0; l < 3; l++) {
+                const Cd x = h.at(i, l, a);
+                const Cd y = h.at(l, j, b);
+                acc.re += x.re * y.re - x.im * y.im;
+                acc.im += x.re * y.im + x.im * y.re;
             }
+            P[i][j] = Cd{acc.re / d, acc.im / d};
+        }
+    const int nm[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+#pragma unroll
+    for (int w = 0; w < 6; w++) {
+        const int n = nm[w][0], m = nm[w][1];
+        Cd acc{0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                double f[4];
+                axfac(p, n, m, i, j, f);
+                acc.re += f[0] * P[i][j].re + f[1] * P[i][j].im;
+                acc.im += f[2] * P[i][j].im + f[3] * P[i][j].re;
+            }
+        if (w < 3) {
+            Q[w] = acc.re;
+        } else {
+            Q[3 + 2 * (w - 3)] = acc.re;
+            Q[4 + 2 * (w - 3)] = acc.im;
         }
     }
+    rate = -kLoE * mv[k][0];
 }
 
 __device__ inline void slot_inputs(const GridDesc& g, int s, int e, double& E, double& rho) {
@@ -475,23 +482,32 @@ __device__ inline void slot_inputs(const GridDesc& g, int s, int e, double& E, d
 
 // FP64 table, per (slot s, energy e): all three Q_k (Q[(s*27 + k*9 + f)*ne_pad + e]) and the
 // phase coefficients R[(s*3 + k)*ne_pad + e] = -2.534 lambda_k (the per-pair phase is R*L/E,
-// the reference's expression).
+// the reference's expression).  Each cell is formed by three lanes of three wavefronts, one
+// projector each (wavefront w takes projector w mod 3 of 64 consecutive cells, so the choice is
+// wave-uniform; the cubic and 2EH - M are formed by all three): the kernel is latency-bound, and
+// one lane forming all three projectors took three times the dependency chain and 256
+// registers.  ne_pad is a multiple of 64, so the 64 cells of a wavefront share the slot.
 __global__ __launch_bounds__(kBlock) void shell_spectrum_f64(GridDesc g, OscParams p, int ne_pad,
                                                              double* __restrict__ Q, double* __restrict__ R) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (g.rows + 1) * ne_pad) return;
-    const int s = t / ne_pad, e = t % ne_pad;
+    const int w = t / kWave;
+    const int k = __builtin_amdgcn_readfirstlane(w % 3);
+    const int cell = (w / 3) * kWave + t % kWave;
+    if (cell >= (g.rows + 1) * ne_pad) return;
+    const int s = cell / ne_pad, e = cell % ne_pad;
     double E, rho;
     slot_inputs(g, s, e, E, rho);
-    double q[3][9], mm[3][3], mv[3][3];
-    flavour_projectors(p, E, rho, q, mm, mv);
+    double q[9], rate;
+    if (k == 0)
+        flavour_projector<0>(p, E, rho, q, rate);
+    else if (k == 1)
+        flavour_projector<1>(p, E, rho, q, rate);
+    else
+        flavour_projector<2>(p, E, rho, q, rate);
     const std::size_t st = ne_pad;
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-#pragma unroll
-        for (int f = 0; f < 9; f++) Q[(std::size_t(s) * 27 + k * 9 + f) * st + e] = q[k][f];
-        R[(std::size_t(s) * 3 + k) * st + e] = -kLoE * mv[k][0];
-    }
+    for (int f = 0; f < 9; f++) Q[(std::size_t(s) * 27 + k * 9 + f) * st + e] = q[f];
+    R[(std::size_t(s) * 3 + k) * st + e] = rate;
 }
 
 // Mixed-precision table, per (slot s, energy e), in the real basis of OscParams::Hr / Wmu.
@@ -701,7 +717,7 @@ hipError_t launch_batch_setup(const BatchDesc& b, int blocks, hipStream_t s) {
 
 hipError_t launch_spectrum_f64(const GridDesc& g, const OscParams& p, int ne_pad, double* Q, double* R,
                                hipStream_t s) {
-    const unsigned n = unsigned(g.rows + 1) * ne_pad;
+    const unsigned n = 3u * unsigned(g.rows + 1) * ne_pad;
     if (n == 0 || g.n_e == 0) return hipSuccess;
     hipLaunchKernelGGL(shell_spectrum_f64, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g, p, ne_pad, Q,
                        R);
