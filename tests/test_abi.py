@@ -42,6 +42,14 @@ def test_null_handle_is_rejected_without_device_work():
     out = ctypes.c_void_p()
     rc = lib.cp3h_create(0, 4, 4, 7, ctypes.byref(out))
     assert rc == 1 and "unknown precision mode" in lib.cp3h_last_error().decode()
+    su, pa = ctypes.c_float(), ctypes.c_float()
+    rc = lib.cp3h_time_kernels(None, 0, 0.0, None, 5, ctypes.byref(su), ctypes.byref(pa))
+    assert rc == 1 and "null propagator handle" in lib.cp3h_last_error().decode()
+    ptr, nbytes = ctypes.c_void_p(), ctypes.c_int()
+    rc = lib.cp3h_fetch_results(None, ctypes.byref(ptr), ctypes.byref(nbytes))
+    assert rc == 1 and "null propagator handle" in lib.cp3h_last_error().decode()
+    rc = lib.cp3h_batch_time_kernels(None, 5, ctypes.byref(su), ctypes.byref(pa))
+    assert rc == 1 and "null batch handle" in lib.cp3h_last_error().decode()
 
 
 def test_cpp_headers_compile():
