@@ -715,14 +715,26 @@ def main():
         return
 
     result, grids, gpu_results = run_jitter12(args, sync)
+
+    def sub_object(key, fn):
+        # a failing sub-object is reported in its place (and on stderr) instead of taking the
+        # headline line with it
+        try:
+            result[key] = fn()
+        except Exception as e:          # noqa: BLE001 -- recorded, not swallowed
+            import traceback
+            traceback.print_exc()
+            result[key] = {"error": f"{type(e).__name__}: {e}"}
+
     if args.precision == "mixed":
         if not args.no_surrogate:
-            result["surrogate"] = surrogate_comparison(args, sync)
+            sub_object("surrogate", lambda: surrogate_comparison(args, sync))
         if not args.no_scan:
-            result["scan1024"] = run_scan(args, sync, args.sub_steps, args.sub_warmup, cpu=not args.no_cpu_baseline)
+            sub_object("scan1024", lambda: run_scan(args, sync, args.sub_steps, args.sub_warmup,
+                                                    cpu=not args.no_cpu_baseline))
         if not args.no_mcmc:
-            result["mcmc8192"] = run_mcmc(args, sync, args.sub_steps, args.sub_warmup, args.binned_chain_steps,
-                                          cpu=not args.no_cpu_baseline)
+            sub_object("mcmc8192", lambda: run_mcmc(args, sync, args.sub_steps, args.sub_warmup,
+                                                    args.binned_chain_steps, cpu=not args.no_cpu_baseline))
     if not args.no_cpu_baseline:
         threads = host_threads()
         log(f"[bench] CPU baseline with {threads} threads ...")
