@@ -74,23 +74,38 @@ def make_prop(g, precision):
     return p
 
 
-def time_steps(props, grids, steps, warmup, sync, after_warmup=None):
-    """One step = every variant's calculateProbabilitiesAsync (parameter hand-over + launches, in
-    order, on the propagators' stream), then waitForCompletion on each.  after_warmup() runs
-    between the warm-up and the timed steps (untimed)."""
-    def step():
+def time_steps(props, grids, steps, warmup, sync, after_warmup=None, schedule="rolling"):
+    """One step = one calculateProbabilitiesAsync (parameter hand-over + launches on the
+    propagator's stream) and one waitForCompletion of every variant.  schedule "grouped": all
+    launched, then all awaited, before the next step; "rolling": each propagator is awaited and
+    relaunched in turn, so it starts its next computation as soon as its previous one is done
+    while the others still run (every computation is still awaited before its propagator is used
+    again, and all are awaited at the end).  after_warmup() runs between the warm-up and the
+    timed steps (untimed)."""
+    def grouped_step():
         for p, g in zip(props, grids):
             p.calculateProbabilitiesAsync(g.antinu)
         for p in props:
             p.waitForCompletion()
+
+    def rolling_step():
+        for p, g in zip(props, grids):
+            p.waitForCompletion()
+            p.calculateProbabilitiesAsync(g.antinu)
+
+    step = rolling_step if schedule == "rolling" else grouped_step
     for _ in range(warmup):
         step()
+    for p in props:
+        p.waitForCompletion()
     if after_warmup is not None:
         after_warmup()
     sync()
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
+    for p in props:
+        p.waitForCompletion()
     sync()
     return time.perf_counter() - t0
 
@@ -205,9 +220,13 @@ def run_jitter12(args, sync):
     times = {}
     dt = time_steps(props, grids, args.steps, args.warmup, sync,
                     after_warmup=lambda: times.update(k=kernel_times(props, grids, args.kernel_reps)))
+    # the same steps with all four launched and all four awaited before the next step
+    dt_grouped = time_steps(props, grids, args.steps, 0, sync, schedule="grouped")
     setup_ms, k_ms = times["k"]
     value = args.steps * pairs_per_step / dt
-    log(f"[bench] {args.steps} steps in {dt:.4f} s -> {value:.4e} pairs/s")
+    value_grouped = args.steps * pairs_per_step / dt_grouped
+    log(f"[bench] {args.steps} steps in {dt:.4f} s -> {value:.4e} pairs/s (each propagator relaunched as soon as "
+        f"it is done); {value_grouped:.4e} pairs/s with all four awaited before the next step")
 
     ml = M.maxlayers_of(grids[0].cosines, grids[0].radii, grids[0].rhos)
     ne = len(grids[0].energies)
@@ -259,7 +278,10 @@ def run_jitter12(args, sync):
         "config": {"workload": f"{workload}: {args.n} cosZ in [-1,1] x {args.n} E log 0.05-1000 GeV, "
                                "12 jittered PREM shells, h=22 km, nu/nubar x NO/IO (4 grids per step)",
                    "precision": args.precision, "pairs_per_step": pairs_per_step,
-                   "streams": "one shared" if args.shared_stream else "one per propagator"},
+                   "streams": "one shared" if args.shared_stream else "one per propagator",
+                   "steps": "each propagator awaited and relaunched in turn (rolling); every computation "
+                            "is awaited before its propagator is reused and all at the end"},
+        "value_grouped_steps": value_grouped,
         "roofline": roof,
         "d2h": {"ms_per_grid": d2h_ms, "bytes_per_grid": d2h_bytes, "GBps": d2h_bytes / (d2h_ms * 1e-3) / 1e9,
                 "pairs_per_s_including_d2h": grids[0].n_pairs / ((dt / args.steps / len(grids)) + d2h_ms * 1e-3),
