@@ -672,7 +672,9 @@ def main():
                     help="back-to-back launches per variant for the kernel durations of the jitter12 line")
     ap.add_argument("--kernel-reps-batch", type=int, default=5,
                     help="back-to-back launches for the kernel durations of the scan1024 and mcmc8192 lines")
-    ap.add_argument("--sub-steps", type=int, default=20, help="timed steps of the scan1024 / mcmc8192 sub-objects")
+    ap.add_argument("--sub-steps", type=int, default=20, help="timed steps of the scan1024 sub-object")
+    ap.add_argument("--chain-steps", type=int, default=10000,
+                    help="timed steps of the mcmc8192 sub-object (the workload's chain of 10,000 proposals)")
     ap.add_argument("--sub-warmup", type=int, default=3)
     ap.add_argument("--shared-stream", action="store_true",
                     help="jitter12: run the four propagators on one stream (default: each on its own)")
@@ -755,7 +757,7 @@ def main():
             sub_object("scan1024", lambda: run_scan(args, sync, args.sub_steps, args.sub_warmup,
                                                     cpu=not args.no_cpu_baseline))
         if not args.no_mcmc:
-            sub_object("mcmc8192", lambda: run_mcmc(args, sync, args.sub_steps, args.sub_warmup,
+            sub_object("mcmc8192", lambda: run_mcmc(args, sync, args.chain_steps, args.sub_warmup,
                                                     args.binned_chain_steps, cpu=not args.no_cpu_baseline))
     if not args.no_cpu_baseline:
         threads = host_threads()
