@@ -158,6 +158,30 @@ def accuracy_vs_fp64_kernel(grids, props):
     return worst, float(np.sqrt(sq / n))
 
 
+VALU_CEILING: dict = {}
+
+
+def measure_valu_ceiling() -> dict:
+    """The sustained non-packed FP32 / FP64 FMA rates of the GPU (cp3h_measure_valu_peak), measured
+    once per run: the ceiling of kernels that issue one FMA per instruction, reported beside the
+    data-sheet peak."""
+    from cudaprob3_amd import _lib
+    f32, f64 = ctypes.c_float(), ctypes.c_float()
+    _lib.check(_lib.load().cp3h_measure_valu_peak(0, ctypes.byref(f32), ctypes.byref(f64)))
+    VALU_CEILING.update({"fp32_fma_TFLOPs": f32.value, "fp64_fma_TFLOPs": f64.value})
+    return VALU_CEILING
+
+
+def with_valu_ceiling(roof: dict, arith: str = "fp32") -> dict:
+    """Adds the measured FMA ceiling of `arith` ("fp32" or "fp64"), and the fraction of it the
+    kernel reaches, to a roofline entry."""
+    key = f"{arith}_fma_TFLOPs"
+    if VALU_CEILING and roof.get("achieved"):
+        roof["measured_fma_ceiling_TFLOPs"] = VALU_CEILING[key]
+        roof["frac_of_measured_fma_ceiling"] = roof["achieved"] / VALU_CEILING[key]
+    return roof
+
+
 def traffic_from_profile(workload, precision, key="hbm_bytes_per_launch"):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{precision}.json")
     if not os.path.exists(path):
@@ -282,7 +306,7 @@ def run_jitter12(args, sync):
                    "steps": "each propagator awaited and relaunched in turn (rolling); every computation "
                             "is awaited before its propagator is reused and all at the end"},
         "value_grouped_steps": value_grouped,
-        "roofline": roof,
+        "roofline": with_valu_ceiling(roof, "fp32" if args.precision == "mixed" else "fp64"),
         "d2h": {"ms_per_grid": d2h_ms, "bytes_per_grid": d2h_bytes, "GBps": d2h_bytes / (d2h_ms * 1e-3) / 1e9,
                 "pairs_per_s_including_d2h": grids[0].n_pairs / ((dt / args.steps / len(grids)) + d2h_ms * 1e-3),
                 "note": "results stay in HBM in the timed steps; this is the lazy copy-back of getProbability"},
@@ -299,17 +323,28 @@ def run_jitter12(args, sync):
                 p.calculateProbabilities(g.antinu)
                 gpu_results.append(p.getProbabilities().copy())
     if not args.no_extra:
-        extra = {}
+        extra, extra_roof = {}, {}
         for prec in ("fp64", "fp64_plain"):
             if prec == args.precision:
                 continue
             qs = [make_prop(g, prec) for g in grids]
             dtq = time_steps(qs, grids, 2, 1, sync)
             extra[prec] = 2 * pairs_per_step / dtq
-            log(f"[bench] {prec}: {extra[prec]:.4e} pairs/s")
+            # the FP64 pair kernels against the FP64 vector peak, on the flops of the algorithm each
+            # runs (the plain kernel's is the reference's own)
+            su, ka = kernel_times(qs, grids, 5, preroll_s=0.1)
+            flops = ref_flops if prec == "fp64_plain" else float(M.f64_hoisted_flops_per_pair(ml).sum()) * ne
+            a = flops / (ka * 1e-3) / 1e12
+            extra_roof[prec] = with_valu_ceiling(
+                {"kernel": "propagate_plain_f64" if prec == "fp64_plain" else "pairs_f64_kernel",
+                 "kernel_ms": ka, "setup_kernel_ms": su, "flops_per_launch": flops, "achieved": a,
+                 "peak": M.PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s", "frac": a / M.PEAK_FP64_VALU_TFLOPS}, "fp64")
+            log(f"[bench] {prec}: {extra[prec]:.4e} pairs/s; pair kernel {ka:.3f} ms, {a:.1f} TFLOP/s = "
+                f"{a / M.PEAK_FP64_VALU_TFLOPS:.3f} of the FP64 vector peak")
             for q in qs:
                 q.close()
         result["other_precisions_pairs_per_s"] = extra
+        result["other_precisions_roofline"] = extra_roof
     for p in props:
         p.close()
     del shared
@@ -370,7 +405,7 @@ def run_scan(args, sync, steps, warmup, cpu=True):
                                "random Earths, new hypotheses every step, HIP-graph replay, steps pipelined "
                                "(host writes step k+1 while the device runs step k)",
                    "pairs_per_step": b.n_pairs, "autotuned": {"waves_per_group": waves, "groups_per_cu": gpc}},
-        "roofline": roof,
+        "roofline": with_valu_ceiling(roof),
     }
     if cpu:
         threads = host_threads()
@@ -423,12 +458,12 @@ def run_mcmc(args, sync, steps, warmup, binned_steps, cpu=True):
                    "pairs_per_step": chain.n_pairs},
         "latency_ms": {"p50": p50, "p99": p99, "mean": float(np.mean(st.step_ms))},
         "acceptance": st.accepted / max(st.steps, 1),
-        "roofline": {"bound": "valu", **M.valu_roofline(f32, f64, k_ms),
+        "roofline": with_valu_ceiling({"bound": "valu", **M.valu_roofline(f32, f64, k_ms),
                      "traffic": traffic_from_profile("mcmc8192", "mixed"),
                      "counter_flops_per_launch": traffic_from_profile("mcmc8192", "mixed", "counter_flops_per_launch"),
                      "kernel": "pairs_mixed_kernel<true>",
                      "kernel_ms": k_ms, "setup_kernel_ms": setup_ms, "launch_reps": args.kernel_reps_batch,
-                     "reference_work_TFLOPs": ref / (k_ms * 1e-3) / 1e12},
+                     "reference_work_TFLOPs": ref / (k_ms * 1e-3) / 1e12}),
     }
     chain.close()
     if cpu:
@@ -715,6 +750,9 @@ def main():
         sync()
         log(f"[bench] calibration copy: {src.numel() * 4} bytes read, {dst.numel() * 4} bytes written")
         del src, dst
+    ceiling = measure_valu_ceiling()
+    log(f"[bench] measured FMA ceiling (non-packed, 8 wavefronts per SIMD): FP32 {ceiling['fp32_fma_TFLOPs']:.1f}, "
+        f"FP64 {ceiling['fp64_fma_TFLOPs']:.1f} TFLOP/s")
 
     if args.workload == "scan1024":
         print(json.dumps(run_scan(args, sync, args.steps, args.warmup, cpu=not args.no_cpu_baseline)), flush=True)

@@ -55,6 +55,7 @@ SIGNATURES = {
     "cp3h_set_timing": (ctypes.c_int, [_handle, ctypes.c_int]),
     "cp3h_time_kernels": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_int,
                                          _c_float_p, _c_float_p]),
+    "cp3h_measure_valu_peak": (ctypes.c_int, [ctypes.c_int, _c_float_p, _c_float_p]),
     "cp3h_last_error": (ctypes.c_char_p, []),
     "cp3h_propagate_mixed_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                                     ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,

@@ -147,6 +147,15 @@ int cp3h_set_timing(cp3h_propagator* h, int enable);
 int cp3h_time_kernels(cp3h_propagator* h, int antineutrino, double production_height_cm, const float* data_device,
                       int reps, float* setup_ms, float* pairs_ms);
 
+/* Roofline calibration (no counterpart in the reference): the sustained rate, in TFLOP/s, of
+   plain (non-packed) FP32 and FP64 vector fused multiply-adds on `device`: independent FMA
+   chains in every lane, eight wavefronts per SIMD; the best of five launches of about 2 ms, each
+   between HIP events, after 40 untimed launches that bring the clocks up (about 0.15 s in all).  It is the ceiling of kernels whose arithmetic issues
+   one FMA per instruction, as this library's do (packed FP32 is disabled), beside the data
+   sheet's FP32 figure, which counts packed FP32.  Allocates and frees a small device buffer;
+   either pointer may be NULL to skip that precision. */
+int cp3h_measure_valu_peak(int device, float* fp32_tflops, float* fp64_tflops);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char* cp3h_last_error(void);
 

@@ -50,6 +50,8 @@ def test_null_handle_is_rejected_without_device_work():
     assert rc == 1 and "null propagator handle" in lib.cp3h_last_error().decode()
     rc = lib.cp3h_batch_time_kernels(None, 5, ctypes.byref(su), ctypes.byref(pa))
     assert rc == 1 and "null batch handle" in lib.cp3h_last_error().decode()
+    rc = lib.cp3h_measure_valu_peak(0, None, None)
+    assert rc == 1 and "null output pointers" in lib.cp3h_last_error().decode()
 
 
 def test_cpp_headers_compile():

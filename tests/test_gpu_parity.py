@@ -324,3 +324,18 @@ def test_float32_api_runs_mixed(oracle_mod):
         ref = oracle_mod.propagate(0, cos.astype(np.float64), E.astype(np.float64), g.radii.astype(np.float32),
                                    g.rhos.astype(np.float32), 22.0, params)
         assert np.abs(P - ref).max() <= 5e-4   # float32 inputs and float32 host set-up dominate
+
+
+def test_measured_fma_ceiling_is_plausible():
+    """cp3h_measure_valu_peak: the sustained non-packed FMA rates lie below the data sheet's
+    FP32 (157.3, packed) and FP64 (78.6) vector peaks and above a fifth of the non-packed rates, and
+    the pair kernel's figure in bench.py is a fraction of them."""
+    import ctypes
+    from cudaprob3_amd import _lib
+    lib = _lib.load()
+    f32, f64 = ctypes.c_float(), ctypes.c_float()
+    assert lib.cp3h_measure_valu_peak(0, ctypes.byref(f32), ctypes.byref(f64)) == 0, lib.cp3h_last_error()
+    print(f"[gpu] measured FMA ceiling: FP32 {f32.value:.1f} TFLOP/s, FP64 {f64.value:.1f} TFLOP/s")
+    assert 78.6 / 5 < f32.value < 157.3
+    assert 78.6 / 5 < f64.value < 78.6 * 1.05
+    assert lib.cp3h_measure_valu_peak(99, ctypes.byref(f32), None) == 3     # CP3H_ERR_DEVICE
