@@ -36,38 +36,41 @@
 #include "common.hpp"
 #include "pair_mixed.hpp"
 #include "reduce.hpp"
+#include "sincos_f64.hpp"
 
 namespace cp3 {
 namespace {
 
-// A = sum_k (c_k + i s_k) Q_k for Hermitian Q_k packed as (Q00, Q11, Q22, Q01, Q02, Q12).
+// A (+)= (c + i s) Q for one Hermitian projector Q packed as (Q00, Q11, Q22, Re/Im Q01, Re/Im Q02,
+// Re/Im Q12); first: A = instead of A +=.  Applied for k = 0, 1, 2 in turn, each entry of A is the
+// sum over k in that order.
 template <typename T>
-__device__ inline void assemble(const T q[27], const T c[3], const T s[3], Cx<T> A[3][3]) {
+__device__ inline void accumulate_projector(const T q[9], T c, T s, bool first, Cx<T> A[3][3]) {
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        T re = c[0] * q[d], im = s[0] * q[d];
-#pragma unroll
-        for (int k = 1; k < 3; k++) {
-            re += c[k] * q[k * 9 + d];
-            im += s[k] * q[k * 9 + d];
+        if (first) {
+            A[d][d] = Cx<T>{c * q[d], s * q[d]};
+        } else {
+            A[d][d].re += c * q[d];
+            A[d][d].im += s * q[d];
         }
-        A[d][d] = Cx<T>{re, im};
     }
     const int idx[3][2] = {{0, 1}, {0, 2}, {1, 2}};
 #pragma unroll
     for (int w = 0; w < 3; w++) {
-        T ure = 0, uim = 0, lre = 0, lim = 0;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const T a = q[k * 9 + 3 + 2 * w], b = q[k * 9 + 4 + 2 * w];
-            const T ca = c[k] * a, sb = s[k] * b, cb = c[k] * b, sa = s[k] * a;
-            ure += ca - sb;
-            uim += cb + sa;
-            lre += ca + sb;
-            lim += sa - cb;
+        const T a = q[3 + 2 * w], b = q[4 + 2 * w];
+        const T ca = c * a, sb = s * b, cb = c * b, sa = s * a;
+        Cx<T>& u = A[idx[w][0]][idx[w][1]];
+        Cx<T>& l = A[idx[w][1]][idx[w][0]];
+        if (first) {
+            u = Cx<T>{ca - sb, cb + sa};
+            l = Cx<T>{ca + sb, sa - cb};
+        } else {
+            u.re += ca - sb;
+            u.im += cb + sa;
+            l.re += ca + sb;
+            l.im += sa - cb;
         }
-        A[idx[w][0]][idx[w][1]] = Cx<T>{ure, uim};
-        A[idx[w][1]][idx[w][0]] = Cx<T>{lre, lim};
     }
 }
 
@@ -128,16 +131,18 @@ __global__ __launch_bounds__(kBlock) void pairs_f64_kernel(PairArgs a) {
         const double dist = gl[i];
         const T* q = Q + std::size_t(i) * 27 * st + ie;
         const double* r = a.R + std::size_t(i) * 3 * st + ie;
-        T qv[27];
-#pragma unroll
-        for (int f = 0; f < 27; f++) qv[f] = q[f * st];
-        T c[3], s[3];
+        // A = sum_k (c_k + i s_k) Q_k, one projector at a time (the same sums, in the same order,
+        // as forming all three phases first; fewer values live at once)
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const double arg = r[k * st] * dist / E;
-            sincos(arg, &s[k], &c[k]);
+            T ck, sk;
+            if (!sincos_reduced_f64(arg, sk, ck)) sincos(arg, &sk, &ck);
+            T qk[9];
+#pragma unroll
+            for (int f = 0; f < 9; f++) qk[f] = q[(k * 9 + f) * st];
+            accumulate_projector<T>(qk, ck, sk, k == 0, A);
         }
-        assemble<T>(qv, c, s, A);
         if (i == 0) {
 #pragma unroll
             for (int x = 0; x < 3; x++)
