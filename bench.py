@@ -750,9 +750,12 @@ def main():
         sync()
         log(f"[bench] calibration copy: {src.numel() * 4} bytes read, {dst.numel() * 4} bytes written")
         del src, dst
-    ceiling = measure_valu_ceiling()
-    log(f"[bench] measured FMA ceiling (non-packed, 8 wavefronts per SIMD): FP32 {ceiling['fp32_fma_TFLOPs']:.1f}, "
-        f"FP64 {ceiling['fp64_fma_TFLOPs']:.1f} TFLOP/s")
+    try:
+        ceiling = measure_valu_ceiling()
+        log(f"[bench] measured FMA ceiling (non-packed, 8 wavefronts per SIMD): FP32 {ceiling['fp32_fma_TFLOPs']:.1f}, "
+            f"FP64 {ceiling['fp64_fma_TFLOPs']:.1f} TFLOP/s")
+    except RuntimeError as e:           # the roofline entries then carry the data-sheet peak only
+        log(f"[bench] FMA ceiling not measured: {e}")
 
     if args.workload == "scan1024":
         print(json.dumps(run_scan(args, sync, args.steps, args.warmup, cpu=not args.no_cpu_baseline)), flush=True)
