@@ -84,7 +84,7 @@ __device__ __attribute__((always_inline)) inline void hidden_layer(__bf16* act, 
         const __bf16* arow = act + (wm * kMT * 16 + i * 16 + l16) * kActStride + 8 * q;
         f32x4 acc[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int j = 0; j < 4; j++) acc[j] = bv[j];         // the bias, then the products
 #pragma unroll
         for (int kk = 0; kk < kKS; kk++) {
             const bf16x8 ah = *reinterpret_cast<const bf16x8*>(arow + kk * 32);
@@ -101,7 +101,7 @@ __device__ __attribute__((always_inline)) inline void hidden_layer(__bf16* act, 
         for (int j = 0; j < 4; j++) {
             float g[4];
 #pragma unroll
-            for (int r = 0; r < 4; r++) g[r] = gelu(acc[j][r] + bv[j][r]);
+            for (int r = 0; r < 4; r++) g[r] = gelu(acc[j][r]);
             two_term_pair(g[0], g[1], out_hi[i][j].x, out_lo[i][j].x);
             two_term_pair(g[2], g[3], out_hi[i][j].y, out_lo[i][j].y);
         }
